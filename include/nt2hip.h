@@ -1,0 +1,162 @@
+/* nt2hip.h — C ABI of the MI355X (gfx950) evaluator for NT2 table expressions.
+ *
+ * This is the accelerator slot of NT2's evaluation-site stack
+ * (accelerator_site<shared_memory_site<...>>, modules/sdk/functor/include/
+ * nt2/sdk/functor/site.hpp:50-53): where NT2 would dispatch transform_ and
+ * inner_fold_ to a CUDA site (modules/arch/cuda/include/nt2/sdk/cuda/
+ * cuda.hpp:22-41 registers none), a caller binds these entry points.  Every
+ * entry point evaluates one fused loop nest as one kernel launch.
+ *
+ * Conventions
+ *  - Plain C types only.  Pointers are caller-owned device pointers unless a
+ *    comment says otherwise; sizes are element counts (int64_t).
+ *  - Matrices are column-major (NT2's default storage order,
+ *    core/settings/storage_order.hpp:22) with an explicit leading dimension.
+ *  - All calls are asynchronous on the given stream (NULL = the default
+ *    stream) except where noted.  Returns NT2H_OK (0) or a negative
+ *    nt2h_status; arguments are validated before anything is launched.
+ *  - Reductions are deterministic: fixed combine order, no float atomics.
+ *    Their results (double) may be written to device memory (asynchronous)
+ *    or to ordinary host memory (the call then synchronises the stream).
+ *  - Thread safety: calls on distinct streams may run concurrently; the
+ *    library keeps one reduction workspace per (device, stream).
+ */
+#ifndef NT2HIP_H
+#define NT2HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct nt2h_stream_opaque* nt2h_stream_t; /* a hipStream_t; NULL = default stream */
+
+typedef enum nt2h_status {
+  NT2H_OK = 0,
+  NT2H_ERR_INVALID_ARG = -1,  /* bad enum value, negative size, bad leading dimension */
+  NT2H_ERR_NULL_POINTER = -2, /* a required pointer is NULL while n > 0 */
+  NT2H_ERR_BAD_OP = -3,       /* unsupported functor id for this entry point */
+  NT2H_ERR_NO_DEVICE = -4,    /* no HIP device, or device index out of range */
+  NT2H_ERR_HIP = -5,          /* a HIP runtime call failed (see nt2h_last_hip_error) */
+  NT2H_ERR_OUT_OF_MEMORY = -6,/* device allocation failed */
+  NT2H_ERR_TOO_LARGE = -7     /* size beyond what the entry point supports */
+} nt2h_status;
+
+/* Elementwise functor ids (the NT2 functors on the path). */
+typedef enum nt2h_op {
+  NT2H_OP_EXP = 0,         /* nt2::exp      (exponential/.../expo/f_expo_reduction.hpp:49-85)      */
+  NT2H_OP_LOG = 1,         /* nt2::log      (exponential/.../logs/f_log.hpp:59-87)                 */
+  NT2H_OP_SQRT = 2,        /* nt2::sqrt     (boost/simd/.../scalar/sqrt.hpp:46)                    */
+  NT2H_OP_ERF = 3,         /* nt2::erf      (euler/.../scalar/erf.hpp:76-100)                      */
+  NT2H_OP_ERFC = 4,        /* nt2::erfc     (euler/.../scalar/erfc.hpp:77-105)                     */
+  NT2H_OP_NORMCDF = 5,     /* nt2::normcdf  (statistics/.../generic/normcdf.hpp:34)                */
+  NT2H_OP_TANH = 6,        /* nt2::tanh     (hyperbolic/.../scalar/tanh.hpp:45-54)                 */
+  NT2H_OP_SIN = 7,         /* nt2::sin      (trigonometric/.../generic/sin.hpp:27)                 */
+  NT2H_OP_COS = 8,         /* nt2::cos      (trigonometric/.../generic/cos.hpp:28)                 */
+  NT2H_OP_NORMPDF = 9,     /* nt2::normpdf  (statistics/.../generic/normpdf.hpp:31)                */
+  NT2H_OP_FASTNORMCDF = 10,/* nt2::fastnormcdf, float only (statistics/.../fastnormcdf.hpp:39-54)  */
+  NT2H_OP_EXP2 = 11,       /* nt2::exp2, float only                                               */
+  NT2H_OP_EXP10 = 12       /* nt2::exp10, float only                                              */
+} nt2h_op;
+
+/* Arithmetic of the map kernels: EXACT reproduces NT2's scalar functor
+ * bit for bit (two-rounding fma/horner, IEEE division); FUSED uses FMA and
+ * a refined hardware reciprocal (within a stated ULP bound of EXACT). */
+typedef enum nt2h_precision { NT2H_EXACT = 0, NT2H_FUSED = 1 } nt2h_precision;
+
+/* Summation order of the reductions. */
+typedef enum nt2h_sum_mode {
+  NT2H_SUM_NAIVE = 0,    /* per-lane float accumulation, tree, FP64 cross-block combine            */
+  NT2H_SUM_KAHAN = 1,    /* per-lane TwoSum compensation (dot: + exact product error), same tree    */
+  NT2H_SUM_PAIRWISE = 2  /* per-lane binary-counter pairwise cascade, same tree                    */
+} nt2h_sum_mode;
+
+typedef enum nt2h_copy_kind { NT2H_H2H = 0, NT2H_H2D = 1, NT2H_D2H = 2, NT2H_D2D = 3, NT2H_DEFAULT = 4 } nt2h_copy_kind;
+
+/* ------------------------------------------------------------ runtime
+ * Replaces NT2's CUDA buffer/locality layer: cuda_buffer/allocator
+ * (modules/arch/cuda/include/nt2/sdk/memory/cuda/{allocator,buffer}.hpp:51-85),
+ * pinned_allocator.hpp:89-100 and memory::copy (memory/functions/cuda/copy.hpp:52-65).
+ * Errors return codes instead of CUDA_ERROR's exit(1) (sdk/cuda/cuda.hpp:44-50). */
+int nt2h_init(int device);              /* select device, create per-device state              */
+int nt2h_device_count(int* count);
+const char* nt2h_strerror(int code);
+int nt2h_last_hip_error(void);          /* hipError_t of the last failing HIP call (this thread) */
+int nt2h_malloc(void** ptr, size_t bytes);
+int nt2h_free(void* ptr);
+int nt2h_host_alloc(void** ptr, size_t bytes); /* pinned host memory */
+int nt2h_host_free(void* ptr);
+int nt2h_memcpy(void* dst, const void* src, size_t bytes, int kind, nt2h_stream_t s);
+int nt2h_sync(nt2h_stream_t s);
+
+/* ---------------------------------------------------- elementwise (transform_)
+ * NT2 transform: core/functions/transform.hpp:55-69 -> elementwise run_assign
+ * (dsl/functions/elementwise/run.hpp:21-35), one fused loop nest per assignment. */
+
+/* Y = alpha*X + Y (demo/saxpy/nt2/saxpy_nt2.cpp:41); two roundings, no
+ * contraction: bit-identical to NT2's scalar path.  x and y may not overlap
+ * except x == y. */
+int nt2h_saxpy_f32(float alpha, const float* x, float* y, int64_t n, nt2h_stream_t s);
+
+/* y = op(x) (one functor node of a table expression); y may equal x. */
+int nt2h_map_f32(int op, int precision, const float* x, float* y, int64_t n, nt2h_stream_t s);
+int nt2h_map_f64(int op, int precision, const double* x, double* y, int64_t n, nt2h_stream_t s);
+
+/* Black-Scholes European call and put, one fused loop nest with 5 input and
+ * 2 output streams:
+ *   d1 = (log(S/K) + (r + sqr(sig)/2)*T) / (sig*sqrt(T)), d2 = d1 - sig*sqrt(T)
+ *   call = S*normcdf(d1) - K*exp(-r*T)*normcdf(d2)
+ *   put  = K*exp(-r*T)*normcdf(-d2) - S*normcdf(-d1)
+ * (the formula NT2's blackscholes demo evaluates as four loop nests,
+ * demo/blackscholes/nt2/blackandscholes_nt2.cpp:53-60, with the textbook d1). */
+int nt2h_bs_fwd_f32(int precision, const float* S, const float* K, const float* T, const float* r,
+                    const float* sig, float* call, float* put, int64_t n, nt2h_stream_t s);
+int nt2h_bs_fwd_f64(int precision, const double* S, const double* K, const double* T, const double* r,
+                    const double* sig, double* call, double* put, int64_t n, nt2h_stream_t s);
+
+/* ------------------------------------------------------ reductions (inner_fold_)
+ * NT2 inner_fold with plus/Zero/sum (core/reduction/.../scalar/inner_fold.hpp:24-53,
+ * dsl/functions/reduction/run.hpp:30-104); globalsum (core/functions/globalsum.hpp:45)
+ * and dot (reduction/functions/expr/dot.hpp:28) over a whole table.
+ * Single pass: per-block partials, last-arriving block combines them in FP64
+ * in block order.  *out receives the sum as a double. */
+int nt2h_sum_f32(const float* x, int64_t n, int mode, double* out, nt2h_stream_t s);
+int nt2h_dot_f32(const float* x, const float* y, int64_t n, int mode, double* out, nt2h_stream_t s);
+int nt2h_sum_f64(const double* x, int64_t n, double* out, nt2h_stream_t s);
+int nt2h_dot_f64(const double* x, const double* y, int64_t n, double* out, nt2h_stream_t s);
+
+/* ------------------------------------------------------------- adjoints
+ * NT2 has no differentiation; these pair a forward loop nest with its
+ * hand-written reverse sweep, recomputing forward intermediates in
+ * registers (nothing is stored between the sweeps).
+ *
+ * Black-Scholes call price with cotangent w (portfolio P = sum_i w_i*price_i):
+ *   price_i and the input cotangents dS_i = w_i*dprice/dS_i, dK, dT, dr, dsig;
+ * totals[0] = sum_i dr_i and totals[1] = sum_i dsig_i (dP/dr and dP/dsigma
+ * when r and sigma are one scalar each), compensated, FP64 combine.
+ * totals may be NULL.  Float kernel: FUSED arithmetic.  */
+int nt2h_bs_fwd_adj_f32(const float* S, const float* K, const float* T, const float* r, const float* sig,
+                        const float* w, float* price, float* dS, float* dK, float* dT, float* dr, float* dsig,
+                        double* totals, int64_t n, nt2h_stream_t s);
+int nt2h_bs_fwd_adj_f64(const double* S, const double* K, const double* T, const double* r,
+                        const double* sig, const double* w, double* price, double* dS, double* dK, double* dT,
+                        double* dr, double* dsig, double* totals, int64_t n, nt2h_stream_t s);
+
+/* Column fold of a 2-D table expression and its adjoint, one pass over X:
+ *   z_j = sum_i exp(-X_ij^2/(2 s_j^2)) * cos(k_j X_ij + p_j)
+ * (NT2: sum(exp(-sqr(X)/(2*sqr(sx(s)))) .* cos(...), 1) -> inner_fold over
+ * the rows of each column).  With gz the cotangent of z (NULL = all ones),
+ * dz_ds_j = gz_j*sum e c X^2/s^3, dz_dk_j = -gz_j*sum e sin(.) X,
+ * dz_dp_j = -gz_j*sum e sin(.).  Gradient pointers may all be NULL (forward
+ * only).  X is rows x cols, column-major, leading dimension ld >= rows;
+ * per-column compensated float accumulation. */
+int nt2h_colfold_gc_f32(const float* X, int64_t rows, int64_t cols, int64_t ld, const float* s_,
+                        const float* k_, const float* p_, const float* gz, float* z, float* dz_ds,
+                        float* dz_dk, float* dz_dp, nt2h_stream_t s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NT2HIP_H */

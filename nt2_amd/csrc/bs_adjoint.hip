@@ -1,0 +1,298 @@
+// Black-Scholes forward + adjoint in one pass (the fused kernel whose
+// options/s and HBM fraction are the headline measurement).
+//
+// Forward (per option i): the call price of the table expression
+//   d1 = (log(S/K) + (r + sig^2/2) T) / (sig sqrt T),  d2 = d1 - sig sqrt T
+//   price = S N(d1) - K e^{-rT} N(d2),  N(d) = normcdf(d) = erfc(-d/sqrt2)/2
+// (NT2 functors: f_log.hpp, f_expo_reduction.hpp, erfc.hpp, normcdf.hpp;
+// demo/blackscholes/nt2/blackandscholes_nt2.cpp:53-60 evaluates the same
+// tree, with a drift-only d1, as four loop nests with three temporaries).
+//
+// Reverse sweep with cotangent w_i of price_i, hand-derived and simplified
+// with S phi(d1) = K e^{-rT} phi(d2) (the d1/d2 terms cancel):
+//   dS = w N(d1)                 dK = -w e^{-rT} N(d2)
+//   dT = w (S phi(d1) sig / (2 sqrt T) + r K e^{-rT} N(d2))
+//   dr = w T K e^{-rT} N(d2)     dsig = w S phi(d1) sqrt T
+// Every forward intermediate is recomputed in registers in the same
+// thread; nothing is stored between the sweeps.  phi(d1) reuses the
+// exp(-d1^2/2) that the erfc of N(d1) evaluates anyway.
+//
+// Traffic: 6 input and 6 output streams, 48 B per float option; the two
+// totals sum_i dr_i and sum_i dsig_i (dP/dr, dP/dsigma for scalar r and
+// sigma) are compensated per lane (TwoSum), merged per wave and workgroup,
+// and combined across workgroups in FP64 by the last workgroup.
+#include <hip/hip_runtime.h>
+
+#include "nt2h_common.hpp"
+#include "nt2h_math.hpp"
+
+namespace nt2h {
+
+constexpr float kInvSqrt2 = 0x1.6a09e6p-1f;   // sqrt(1/2)
+constexpr float kInvSqrt2Pi = 0x1.988454p-2f; // 1/sqrt(2 pi)
+
+struct BSAdjOut {
+  float price, dS, dK, dT, dr, dsig;
+};
+
+// One option, float, Fused arithmetic.
+__device__ __forceinline__ BSAdjOut bs_fwd_adj_one(float S, float K, float Tm, float r, float sig, float w) {
+  using P = Fused;
+  float sqT = P::sqrt(Tm);
+  float sst = sig * sqT;
+  float lsk = log_f<P>(P::div(S, K));
+  float d1 = P::div(P::madd(P::madd(sig, sig * 0.5f, r), Tm, lsk), sst);
+  float d2 = d1 - sst;
+  float ert = exp_f<P>(-r * Tm);
+  float kert = K * ert;
+  // N(d1) and exp(-d1^2/2) from one erfc core
+  float t1 = -kInvSqrt2 * d1;
+  float e1;
+  float r1 = erfc_pos<P>(__builtin_fabsf(t1), &e1);
+  float n1 = 0.5f * ((t1 < 0.0f) ? 2.0f - r1 : r1);
+  float t2 = -kInvSqrt2 * d2;
+  float r2 = erfc_pos<P>(__builtin_fabsf(t2));
+  float n2 = 0.5f * ((t2 < 0.0f) ? 2.0f - r2 : r2);
+  float phi1 = e1 * kInvSqrt2Pi;
+  float kn2 = kert * n2;
+  float sphi = S * phi1;
+  BSAdjOut o;
+  o.price = P::madd(S, n1, -kn2);
+  o.dS = w * n1;
+  o.dK = -(w * (ert * n2));
+  o.dT = w * P::madd(sphi * sig, P::div(0.5f, sqT), r * kn2);
+  o.dr = w * (Tm * kn2);
+  o.dsig = w * (sphi * sqT);
+  return o;
+}
+
+// One option, double, Fused (FMA) arithmetic with IEEE division: the plain
+// FP64 evaluation of the same tree, for the precision/speed comparison.
+__device__ __forceinline__ void bs_fwd_adj_one(double S, double K, double Tm, double r, double sig, double w,
+                                               double* o) {
+  using P = Fused;
+  double sqT = __builtin_sqrt(Tm);
+  double sst = sig * sqT;
+  double lsk = log_d<P>(S / K);
+  double d1 = P::madd(P::madd(sig, sig * 0.5, r), Tm, lsk) / sst;
+  double d2 = d1 - sst;
+  double ert = exp_d<P>(-r * Tm);
+  double kert = K * ert;
+  double t1 = -0x1.6a09e667f3bcdp-1 * d1;
+  double e1;
+  double r1 = erfc_pos_d<P>(__builtin_fabs(t1), &e1);
+  double n1 = 0.5 * ((t1 < 0.0) ? 2.0 - r1 : r1);
+  double t2 = -0x1.6a09e667f3bcdp-1 * d2;
+  double r2 = erfc_pos_d<P>(__builtin_fabs(t2));
+  double n2 = 0.5 * ((t2 < 0.0) ? 2.0 - r2 : r2);
+  double phi1 = e1 * 0x1.9884533d43651p-2;
+  double kn2 = kert * n2;
+  double sphi = S * phi1;
+  o[0] = P::madd(S, n1, -kn2);
+  o[1] = w * n1;
+  o[2] = -(w * (ert * n2));
+  o[3] = w * P::madd(sphi * sig, 0.5 / sqT, r * kn2);
+  o[4] = w * (Tm * kn2);
+  o[5] = w * (sphi * sqT);
+}
+
+struct BSAdjArgs {
+  const float* in[6];
+  float* out[6];
+};
+struct BSAdjArgsD {
+  const double* in[6];
+  double* out[6];
+};
+
+template <bool VEC, bool TOTALS>
+__global__ __launch_bounds__(kBlock) void bs_fwd_adj_f32_kernel(BSAdjArgs a, int64_t n, double* partials,
+                                                                unsigned* counter, double* totals) {
+  Comp acc[2] = {{0.0f, 0.0f}, {0.0f, 0.0f}};
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  const int64_t gid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  int64_t done = 0;
+  if (VEC) {
+    const int64_t nv = n / 4;
+    for (int64_t i = gid; i < nv; i += stride) {
+      f32x4 v[6];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) v[k] = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(a.in[k]) + i);
+      f32x4 o[6];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        BSAdjOut r = bs_fwd_adj_one(((const float*)&v[0])[e], ((const float*)&v[1])[e], ((const float*)&v[2])[e],
+                                    ((const float*)&v[3])[e], ((const float*)&v[4])[e], ((const float*)&v[5])[e]);
+        ((float*)&o[0])[e] = r.price;
+        ((float*)&o[1])[e] = r.dS;
+        ((float*)&o[2])[e] = r.dK;
+        ((float*)&o[3])[e] = r.dT;
+        ((float*)&o[4])[e] = r.dr;
+        ((float*)&o[5])[e] = r.dsig;
+        if (TOTALS) {
+          acc[0].add(r.dr);
+          acc[1].add(r.dsig);
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < 6; ++k) __builtin_nontemporal_store(o[k], reinterpret_cast<f32x4*>(a.out[k]) + i);
+    }
+    done = nv * 4;
+  }
+  for (int64_t i = done + gid; i < n; i += stride) {
+    BSAdjOut r = bs_fwd_adj_one(a.in[0][i], a.in[1][i], a.in[2][i], a.in[3][i], a.in[4][i], a.in[5][i]);
+    a.out[0][i] = r.price;
+    a.out[1][i] = r.dS;
+    a.out[2][i] = r.dK;
+    a.out[3][i] = r.dT;
+    a.out[4][i] = r.dr;
+    a.out[5][i] = r.dsig;
+    if (TOTALS) {
+      acc[0].add(r.dr);
+      acc[1].add(r.dsig);
+    }
+  }
+  if (TOTALS) {
+    __shared__ float lds[2 * 2 * kWaves];
+    double bsum[2];
+    block_reduce_comp<2>(acc, lds, bsum);
+    if (threadIdx.x == 0) {
+      partials[blockIdx.x] = bsum[0];
+      partials[gridDim.x + blockIdx.x] = bsum[1];
+    }
+    double res[2];
+    if (grid_combine<2>(partials, counter, res) && threadIdx.x == 0) {
+      totals[0] = res[0];
+      totals[1] = res[1];
+    }
+  }
+}
+
+template <bool TOTALS>
+__global__ __launch_bounds__(kBlock) void bs_fwd_adj_f64_kernel(BSAdjArgsD a, int64_t n, double* partials,
+                                                                unsigned* counter, double* totals) {
+  double acc[2] = {0.0, 0.0};
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    double o[6];
+    bs_fwd_adj_one(__builtin_nontemporal_load(a.in[0] + i), __builtin_nontemporal_load(a.in[1] + i),
+                   __builtin_nontemporal_load(a.in[2] + i), __builtin_nontemporal_load(a.in[3] + i),
+                   __builtin_nontemporal_load(a.in[4] + i), __builtin_nontemporal_load(a.in[5] + i), o);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) __builtin_nontemporal_store(o[k], a.out[k] + i);
+    if (TOTALS) {
+      acc[0] += o[4];
+      acc[1] += o[5];
+    }
+  }
+  if (TOTALS) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+      acc[0] += shfl_xor_d(acc[0], m);
+      acc[1] += shfl_xor_d(acc[1], m);
+    }
+    __shared__ double lds[2 * kWaves];
+    if ((threadIdx.x & 63) == 0) {
+      lds[threadIdx.x >> 6] = acc[0];
+      lds[kWaves + (threadIdx.x >> 6)] = acc[1];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t0 = lds[0], t1 = lds[kWaves];
+      for (int w = 1; w < kWaves; ++w) {
+        t0 += lds[w];
+        t1 += lds[kWaves + w];
+      }
+      partials[blockIdx.x] = t0;
+      partials[gridDim.x + blockIdx.x] = t1;
+    }
+    double res[2];
+    if (grid_combine<2>(partials, counter, res) && threadIdx.x == 0) {
+      totals[0] = res[0];
+      totals[1] = res[1];
+    }
+  }
+}
+
+static unsigned adj_grid(int64_t n, int vec) {
+  int64_t units = (n + vec - 1) / vec;
+  int64_t b = (units + kBlock - 1) / kBlock;
+  if (b < 1) b = 1;
+  if (b > kMaxReduceBlocks) b = kMaxReduceBlocks;
+  return (unsigned)b;
+}
+
+}  // namespace nt2h
+
+using namespace nt2h;
+
+extern "C" {
+
+int nt2h_bs_fwd_adj_f32(const float* S, const float* K, const float* T, const float* r, const float* sig,
+                        const float* w, float* price, float* dS, float* dK, float* dT, float* dr, float* dsig,
+                        double* totals, int64_t n, nt2h_stream_t s) {
+  NT2H_TRY(check_n(n));
+  NT2H_TRY(check_ptrs(n, S, K, T, r, sig, w, price, dS, dK, dT, dr, dsig));
+  Workspace ws{};
+  bool host = false;
+  double* dst = nullptr;
+  if (totals) {
+    NT2H_TRY(is_host_pointer(totals, &host));
+    NT2H_TRY(get_workspace(s, &ws));
+    dst = host ? ws.result : totals;
+  }
+  if (n == 0) {
+    if (totals) {
+      double z[2] = {0.0, 0.0};
+      if (host) { totals[0] = 0.0; totals[1] = 0.0; return NT2H_OK; }
+      NT2H_HIP(hipMemcpyAsync(totals, z, sizeof z, hipMemcpyHostToDevice, as_hip(s)));
+      NT2H_HIP(hipStreamSynchronize(as_hip(s)));
+    }
+    return NT2H_OK;
+  }
+  BSAdjArgs a{{S, K, T, r, sig, w}, {price, dS, dK, dT, dr, dsig}};
+  bool vec = aligned16(S, K, T, r, sig, w, price, dS, dK, dT, dr, dsig);
+  unsigned g = adj_grid(n, vec ? 4 : 1);
+  hipStream_t st = as_hip(s);
+  if (vec) {
+    if (totals) hipLaunchKernelGGL((bs_fwd_adj_f32_kernel<true, true>), dim3(g), dim3(kBlock), 0, st, a, n, ws.partials, ws.counter, dst);
+    else hipLaunchKernelGGL((bs_fwd_adj_f32_kernel<true, false>), dim3(g), dim3(kBlock), 0, st, a, n, ws.partials, ws.counter, dst);
+  } else {
+    if (totals) hipLaunchKernelGGL((bs_fwd_adj_f32_kernel<false, true>), dim3(g), dim3(kBlock), 0, st, a, n, ws.partials, ws.counter, dst);
+    else hipLaunchKernelGGL((bs_fwd_adj_f32_kernel<false, false>), dim3(g), dim3(kBlock), 0, st, a, n, ws.partials, ws.counter, dst);
+  }
+  NT2H_HIP(hipGetLastError());
+  return totals ? deliver(s, totals, ws.result, 2, host) : NT2H_OK;
+}
+
+int nt2h_bs_fwd_adj_f64(const double* S, const double* K, const double* T, const double* r,
+                        const double* sig, const double* w, double* price, double* dS, double* dK, double* dT,
+                        double* dr, double* dsig, double* totals, int64_t n, nt2h_stream_t s) {
+  NT2H_TRY(check_n(n));
+  NT2H_TRY(check_ptrs(n, S, K, T, r, sig, w, price, dS, dK, dT, dr, dsig));
+  Workspace ws{};
+  bool host = false;
+  double* dst = nullptr;
+  if (totals) {
+    NT2H_TRY(is_host_pointer(totals, &host));
+    NT2H_TRY(get_workspace(s, &ws));
+    dst = host ? ws.result : totals;
+  }
+  if (n == 0) {
+    if (totals) {
+      double z[2] = {0.0, 0.0};
+      if (host) { totals[0] = 0.0; totals[1] = 0.0; return NT2H_OK; }
+      NT2H_HIP(hipMemcpyAsync(totals, z, sizeof z, hipMemcpyHostToDevice, as_hip(s)));
+      NT2H_HIP(hipStreamSynchronize(as_hip(s)));
+    }
+    return NT2H_OK;
+  }
+  BSAdjArgsD a{{S, K, T, r, sig, w}, {price, dS, dK, dT, dr, dsig}};
+  unsigned g = adj_grid(n, 1);
+  if (totals) hipLaunchKernelGGL((bs_fwd_adj_f64_kernel<true>), dim3(g), dim3(kBlock), 0, as_hip(s), a, n, ws.partials, ws.counter, dst);
+  else hipLaunchKernelGGL((bs_fwd_adj_f64_kernel<false>), dim3(g), dim3(kBlock), 0, as_hip(s), a, n, ws.partials, ws.counter, dst);
+  NT2H_HIP(hipGetLastError());
+  return totals ? deliver(s, totals, ws.result, 2, host) : NT2H_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,269 @@
+// Elementwise loop nests (NT2 transform_) as single HIP kernels on gfx950.
+//
+// One launch evaluates one fused loop nest: every input stream is read once
+// with 128-bit loads (f32x4 / double2 per lane), the expression tree is
+// evaluated in registers, and every output stream is written once.  A
+// grid-stride loop over 16-byte vectors keeps all 256 CUs busy; a scalar
+// tail (or a scalar kernel when a pointer is not 16-byte aligned) covers the
+// rest.  NT2's counterpart is the transform loop nest (core/functions/
+// transform.hpp:55-69, scalar body expr/transform.hpp:51-59, SIMD body
+// simd/transform.hpp:40-57, OpenMP spawner openmp/spawner/transform.hpp:36-73).
+#include <hip/hip_runtime.h>
+
+#include "nt2h_common.hpp"
+#include "nt2h_math.hpp"
+
+namespace nt2h {
+
+template <class T> struct Vec16;
+template <> struct Vec16<float> { using type = f32x4; static constexpr int n = 4; };
+template <> struct Vec16<double> { using type = f64x2; static constexpr int n = 2; };
+
+template <class V, class T> __device__ __forceinline__ T vget(const V& v, int i) {
+  return reinterpret_cast<const T*>(&v)[i];
+}
+template <class V, class T> __device__ __forceinline__ void vset(V& v, int i, T x) {
+  reinterpret_cast<T*>(&v)[i] = x;
+}
+
+template <class T, int NI, int NO> struct Streams {
+  const T* in[NI];
+  T* out[NO];
+};
+
+// out_k[i] = F(in_0[i], ..., in_{NI-1}[i])_k for i in [0, n).  VEC16 selects
+// the 16-byte vector body (all pointers 16-byte aligned) or the scalar one.
+template <class T, int NI, int NO, bool VEC16, class F>
+__global__ __launch_bounds__(kBlock) void ew_kernel(Streams<T, NI, NO> st, int64_t n, F f) {
+  using VT = typename Vec16<T>::type;
+  constexpr int W = VEC16 ? Vec16<T>::n : 1;
+  const int64_t nv = n / W;
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (VEC16) {
+    for (; i < nv; i += stride) {
+      VT a[NI];
+#pragma unroll
+      for (int k = 0; k < NI; ++k) a[k] = __builtin_nontemporal_load(reinterpret_cast<const VT*>(st.in[k]) + i);
+      VT o[NO];
+#pragma unroll
+      for (int e = 0; e < W; ++e) {
+        T xin[NI], xout[NO];
+#pragma unroll
+        for (int k = 0; k < NI; ++k) xin[k] = vget<VT, T>(a[k], e);
+        f(xin, xout);
+#pragma unroll
+        for (int k = 0; k < NO; ++k) vset<VT, T>(o[k], e, xout[k]);
+      }
+#pragma unroll
+      for (int k = 0; k < NO; ++k) __builtin_nontemporal_store(o[k], reinterpret_cast<VT*>(st.out[k]) + i);
+    }
+    // tail elements [nv*W, n)
+    int64_t t = nv * W + (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t < n) {
+      T xin[NI], xout[NO];
+#pragma unroll
+      for (int k = 0; k < NI; ++k) xin[k] = st.in[k][t];
+      f(xin, xout);
+#pragma unroll
+      for (int k = 0; k < NO; ++k) st.out[k][t] = xout[k];
+    }
+  } else {
+    for (; i < n; i += stride) {
+      T xin[NI], xout[NO];
+#pragma unroll
+      for (int k = 0; k < NI; ++k) xin[k] = st.in[k][i];
+      f(xin, xout);
+#pragma unroll
+      for (int k = 0; k < NO; ++k) st.out[k][i] = xout[k];
+    }
+  }
+}
+
+template <class T, int NI, int NO, class F>
+static int launch_ew(const Streams<T, NI, NO>& st, int64_t n, F f, nt2h_stream_t s) {
+  if (n == 0) return NT2H_OK;
+  bool al = true;
+  for (int k = 0; k < NI; ++k) al = al && aligned16(st.in[k]);
+  for (int k = 0; k < NO; ++k) al = al && aligned16(st.out[k]);
+  if (al) {
+    unsigned g = stream_grid(n, Vec16<T>::n);
+    hipLaunchKernelGGL((ew_kernel<T, NI, NO, true, F>), dim3(g), dim3(kBlock), 0, as_hip(s), st, n, f);
+  } else {
+    unsigned g = stream_grid(n, 1);
+    hipLaunchKernelGGL((ew_kernel<T, NI, NO, false, F>), dim3(g), dim3(kBlock), 0, as_hip(s), st, n, f);
+  }
+  NT2H_HIP(hipGetLastError());
+  return NT2H_OK;
+}
+
+// ------------------------------------------------------------- saxpy
+// demo/saxpy/nt2/saxpy_nt2.cpp:41: Y = alpha*X + Y with two roundings (this
+// file is compiled with -ffp-contract=off, so no FMA is formed).
+struct SaxpyOp {
+  float a;
+  __device__ __forceinline__ void operator()(const float (&in)[2], float (&out)[1]) const {
+    float p = a * in[0];
+    out[0] = p + in[1];
+  }
+};
+
+// ---------------------------------------------------------------- maps
+#define NT2H_MAP_OP(NAME, FN_F, FN_D)                                                          \
+  template <class P> struct NAME {                                                             \
+    __device__ __forceinline__ void operator()(const float (&in)[1], float (&out)[1]) const { \
+      out[0] = FN_F<P>(in[0]);                                                                 \
+    }                                                                                          \
+    __device__ __forceinline__ void operator()(const double (&in)[1], double (&out)[1]) const {\
+      out[0] = FN_D<P>(in[0]);                                                                 \
+    }                                                                                          \
+  };
+NT2H_MAP_OP(OpExp, exp_f, exp_d)
+NT2H_MAP_OP(OpLog, log_f, log_d)
+NT2H_MAP_OP(OpErf, erf_f, erf_d)
+NT2H_MAP_OP(OpErfc, erfc_f, erfc_d)
+NT2H_MAP_OP(OpNormcdf, normcdf_f, normcdf_d)
+NT2H_MAP_OP(OpNormpdf, normpdf_f, normpdf_d)
+NT2H_MAP_OP(OpTanh, tanh_f, tanh_d)
+NT2H_MAP_OP(OpSin, sin_f, sin_d)
+NT2H_MAP_OP(OpCos, cos_f, cos_d)
+#undef NT2H_MAP_OP
+template <class P> struct OpSqrt {
+  __device__ __forceinline__ void operator()(const float (&in)[1], float (&out)[1]) const { out[0] = P::sqrt(in[0]); }
+  __device__ __forceinline__ void operator()(const double (&in)[1], double (&out)[1]) const { out[0] = P::sqrt(in[0]); }
+};
+template <class P> struct OpFastNormcdf {
+  __device__ __forceinline__ void operator()(const float (&in)[1], float (&out)[1]) const { out[0] = fastnormcdf_f<P>(in[0]); }
+};
+template <class P> struct OpExp2 {
+  __device__ __forceinline__ void operator()(const float (&in)[1], float (&out)[1]) const { out[0] = exp2_f<P>(in[0]); }
+};
+template <class P> struct OpExp10 {
+  __device__ __forceinline__ void operator()(const float (&in)[1], float (&out)[1]) const { out[0] = exp10_f<P>(in[0]); }
+};
+
+template <class T, class P>
+static int map_dispatch(int op, const T* x, T* y, int64_t n, nt2h_stream_t s) {
+  Streams<T, 1, 1> st{{x}, {y}};
+  switch (op) {
+    case NT2H_OP_EXP: return launch_ew(st, n, OpExp<P>{}, s);
+    case NT2H_OP_LOG: return launch_ew(st, n, OpLog<P>{}, s);
+    case NT2H_OP_SQRT: return launch_ew(st, n, OpSqrt<P>{}, s);
+    case NT2H_OP_ERF: return launch_ew(st, n, OpErf<P>{}, s);
+    case NT2H_OP_ERFC: return launch_ew(st, n, OpErfc<P>{}, s);
+    case NT2H_OP_NORMCDF: return launch_ew(st, n, OpNormcdf<P>{}, s);
+    case NT2H_OP_TANH: return launch_ew(st, n, OpTanh<P>{}, s);
+    case NT2H_OP_SIN: return launch_ew(st, n, OpSin<P>{}, s);
+    case NT2H_OP_COS: return launch_ew(st, n, OpCos<P>{}, s);
+    case NT2H_OP_NORMPDF: return launch_ew(st, n, OpNormpdf<P>{}, s);
+  }
+  if constexpr (sizeof(T) == 4) {
+    switch (op) {
+      case NT2H_OP_FASTNORMCDF: return launch_ew(st, n, OpFastNormcdf<P>{}, s);
+      case NT2H_OP_EXP2: return launch_ew(st, n, OpExp2<P>{}, s);
+      case NT2H_OP_EXP10: return launch_ew(st, n, OpExp10<P>{}, s);
+    }
+  }
+  return NT2H_ERR_BAD_OP;
+}
+
+static bool op_valid(int op, bool is_float) {
+  if (op >= NT2H_OP_EXP && op <= NT2H_OP_NORMPDF) return true;
+  return is_float && op >= NT2H_OP_FASTNORMCDF && op <= NT2H_OP_EXP10;
+}
+
+// ------------------------------------------------------- Black-Scholes
+// normcdf(d) = 0.5*erfc(-sqrt(1/2)*d) and normcdf(-d) share one erfc core:
+// erfc(-t) = 2 - erfc(t), the same |t| enters both (NT2 evaluates them
+// separately, with identical results).
+template <class P> __device__ __forceinline__ void ncdf_pair(float d, float& nd, float& nmd) {
+  float t = -0x1.6a09e6p-1f * d;
+  float r = erfc_pos<P>(__builtin_fabsf(t));
+  float rr = 2.0f - r;
+  nd = 0.5f * ((t < 0.0f) ? rr : r);
+  nmd = 0.5f * ((t > 0.0f) ? rr : r);
+  if (d != d) { nd = d; nmd = d; }
+}
+template <class P> __device__ __forceinline__ void ncdf_pair(double d, double& nd, double& nmd) {
+  double t = -0x1.6a09e667f3bcdp-1 * d;
+  double r = erfc_pos_d<P>(__builtin_fabs(t));
+  double rr = 2.0 - r;
+  nd = 0.5 * ((t < 0.0) ? rr : r);
+  nmd = 0.5 * ((t > 0.0) ? rr : r);
+  if (d != d) { nd = d; nmd = d; }
+}
+template <class P> __device__ __forceinline__ float logT(float x) { return log_f<P>(x); }
+template <class P> __device__ __forceinline__ double logT(double x) { return log_d<P>(x); }
+template <class P> __device__ __forceinline__ float expT(float x) { return exp_f<P>(x); }
+template <class P> __device__ __forceinline__ double expT(double x) { return exp_d<P>(x); }
+
+template <class T, class P> struct BSFwdOp {
+  __device__ __forceinline__ void operator()(const T (&in)[5], T (&out)[2]) const {
+    const T S = in[0], K = in[1], Tm = in[2], r = in[3], sig = in[4];
+    T sqT = P::sqrt(Tm);
+    T sst = sig * sqT;
+    T d1;
+    if (P::fused) d1 = P::div(P::madd(P::madd(sig, sig * T(0.5), r), Tm, logT<P>(P::div(S, K))), sst);
+    else d1 = P::div(logT<P>(P::div(S, K)) + (r + (sig * sig) * T(0.5)) * Tm, sst);
+    T d2 = d1 - sst;
+    T kert = K * expT<P>(-r * Tm);
+    T n1, nm1, n2, nm2;
+    ncdf_pair<P>(d1, n1, nm1);
+    ncdf_pair<P>(d2, n2, nm2);
+    out[0] = S * n1 - kert * n2;
+    out[1] = kert * nm2 - S * nm1;
+  }
+};
+
+}  // namespace nt2h
+
+using namespace nt2h;
+
+extern "C" {
+
+int nt2h_saxpy_f32(float alpha, const float* x, float* y, int64_t n, nt2h_stream_t s) {
+  NT2H_TRY(check_n(n));
+  NT2H_TRY(check_ptrs(n, x, y));
+  Streams<float, 2, 1> st{{x, y}, {y}};
+  return launch_ew(st, n, SaxpyOp{alpha}, s);
+}
+
+int nt2h_map_f32(int op, int precision, const float* x, float* y, int64_t n, nt2h_stream_t s) {
+  NT2H_TRY(check_n(n));
+  if (!op_valid(op, true)) return NT2H_ERR_BAD_OP;
+  if (precision != NT2H_EXACT && precision != NT2H_FUSED) return NT2H_ERR_INVALID_ARG;
+  NT2H_TRY(check_ptrs(n, x, y));
+  return precision == NT2H_EXACT ? map_dispatch<float, Exact>(op, x, y, n, s)
+                                 : map_dispatch<float, Fused>(op, x, y, n, s);
+}
+
+int nt2h_map_f64(int op, int precision, const double* x, double* y, int64_t n, nt2h_stream_t s) {
+  NT2H_TRY(check_n(n));
+  if (!op_valid(op, false)) return NT2H_ERR_BAD_OP;
+  if (precision != NT2H_EXACT && precision != NT2H_FUSED) return NT2H_ERR_INVALID_ARG;
+  NT2H_TRY(check_ptrs(n, x, y));
+  return precision == NT2H_EXACT ? map_dispatch<double, Exact>(op, x, y, n, s)
+                                 : map_dispatch<double, Fused>(op, x, y, n, s);
+}
+
+int nt2h_bs_fwd_f32(int precision, const float* S, const float* K, const float* T, const float* r,
+                    const float* sig, float* call, float* put, int64_t n, nt2h_stream_t s) {
+  NT2H_TRY(check_n(n));
+  if (precision != NT2H_EXACT && precision != NT2H_FUSED) return NT2H_ERR_INVALID_ARG;
+  NT2H_TRY(check_ptrs(n, S, K, T, r, sig, call, put));
+  Streams<float, 5, 2> st{{S, K, T, r, sig}, {call, put}};
+  return precision == NT2H_EXACT ? launch_ew(st, n, BSFwdOp<float, Exact>{}, s)
+                                 : launch_ew(st, n, BSFwdOp<float, Fused>{}, s);
+}
+
+int nt2h_bs_fwd_f64(int precision, const double* S, const double* K, const double* T, const double* r,
+                    const double* sig, double* call, double* put, int64_t n, nt2h_stream_t s) {
+  NT2H_TRY(check_n(n));
+  if (precision != NT2H_EXACT && precision != NT2H_FUSED) return NT2H_ERR_INVALID_ARG;
+  NT2H_TRY(check_ptrs(n, S, K, T, r, sig, call, put));
+  Streams<double, 5, 2> st{{S, K, T, r, sig}, {call, put}};
+  return precision == NT2H_EXACT ? launch_ew(st, n, BSFwdOp<double, Exact>{}, s)
+                                 : launch_ew(st, n, BSFwdOp<double, Fused>{}, s);
+}
+
+}  // extern "C"

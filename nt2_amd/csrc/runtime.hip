@@ -1,0 +1,161 @@
+// Runtime layer of libnt2hip: device selection, buffers, copies, status
+// strings and the per-(device, stream) reduction workspace.
+//
+// Stands in for NT2's CUDA buffer/locality layer (modules/arch/cuda/include/
+// nt2/sdk/memory/cuda/{allocator,buffer,pinned_allocator}.hpp,
+// memory/functions/cuda/copy.hpp:52-65), with status codes in place of
+// CUDA_ERROR's fprintf+exit (sdk/cuda/cuda.hpp:44-50).
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <mutex>
+#include <utility>
+
+#include "nt2h_common.hpp"
+
+namespace nt2h {
+
+static thread_local int g_last_hip_error = 0;
+void set_hip_error(hipError_t e) { g_last_hip_error = (int)e; }
+
+namespace {
+std::mutex g_ws_mutex;
+std::map<std::pair<int, hipStream_t>, Workspace> g_ws;
+}  // namespace
+
+int get_workspace(nt2h_stream_t s, Workspace* ws) {
+  int dev = 0;
+  NT2H_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(g_ws_mutex);
+  auto key = std::make_pair(dev, as_hip(s));
+  auto it = g_ws.find(key);
+  if (it != g_ws.end()) {
+    *ws = it->second;
+    return NT2H_OK;
+  }
+  Workspace w{};
+  size_t np = (size_t)kMaxReduceBlocks * kMaxPartials * sizeof(double);
+  NT2H_HIP(hipMalloc((void**)&w.partials, np));
+  NT2H_HIP(hipMalloc((void**)&w.counter, 256));
+  NT2H_HIP(hipMalloc((void**)&w.result, kMaxPartials * sizeof(double)));
+  // zero the counter once, synchronously; kernels reset it after each use
+  NT2H_HIP(hipMemset(w.counter, 0, 256));
+  g_ws[key] = w;
+  *ws = w;
+  return NT2H_OK;
+}
+
+int is_host_pointer(const void* p, bool* host) {
+  hipPointerAttribute_t attr;
+  hipError_t e = hipPointerGetAttributes(&attr, p);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();  // unregistered host memory: clear the sticky error
+    *host = true;
+    return NT2H_OK;
+  }
+  *host = !(attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged);
+  return NT2H_OK;
+}
+
+int deliver(nt2h_stream_t s, double* out, const double* dev, int nv, bool out_is_host) {
+  if (!out_is_host) return NT2H_OK;  // kernel wrote out directly
+  NT2H_HIP(hipMemcpyAsync(out, dev, nv * sizeof(double), hipMemcpyDeviceToHost, as_hip(s)));
+  NT2H_HIP(hipStreamSynchronize(as_hip(s)));
+  return NT2H_OK;
+}
+
+}  // namespace nt2h
+
+using namespace nt2h;
+
+extern "C" {
+
+int nt2h_init(int device) {
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count == 0) {
+    (void)hipGetLastError();
+    return NT2H_ERR_NO_DEVICE;
+  }
+  if (device < 0 || device >= count) return NT2H_ERR_NO_DEVICE;
+  NT2H_HIP(hipSetDevice(device));
+  NT2H_HIP(hipFree(nullptr));  // create the context now rather than on first launch
+  return NT2H_OK;
+}
+
+int nt2h_device_count(int* count) {
+  if (!count) return NT2H_ERR_NULL_POINTER;
+  hipError_t e = hipGetDeviceCount(count);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    *count = 0;
+  }
+  return NT2H_OK;
+}
+
+const char* nt2h_strerror(int code) {
+  switch (code) {
+    case NT2H_OK: return "success";
+    case NT2H_ERR_INVALID_ARG: return "invalid argument";
+    case NT2H_ERR_NULL_POINTER: return "null pointer";
+    case NT2H_ERR_BAD_OP: return "unsupported functor id";
+    case NT2H_ERR_NO_DEVICE: return "no HIP device";
+    case NT2H_ERR_HIP: return "HIP runtime error";
+    case NT2H_ERR_OUT_OF_MEMORY: return "out of device memory";
+    case NT2H_ERR_TOO_LARGE: return "size too large";
+  }
+  return "unknown status";
+}
+
+int nt2h_last_hip_error(void) { return g_last_hip_error; }
+
+int nt2h_malloc(void** ptr, size_t bytes) {
+  if (!ptr) return NT2H_ERR_NULL_POINTER;
+  *ptr = nullptr;
+  if (bytes == 0) return NT2H_OK;
+  NT2H_HIP(hipMalloc(ptr, bytes));
+  return NT2H_OK;
+}
+
+int nt2h_free(void* ptr) {
+  if (!ptr) return NT2H_OK;
+  NT2H_HIP(hipFree(ptr));
+  return NT2H_OK;
+}
+
+int nt2h_host_alloc(void** ptr, size_t bytes) {
+  if (!ptr) return NT2H_ERR_NULL_POINTER;
+  *ptr = nullptr;
+  if (bytes == 0) return NT2H_OK;
+  NT2H_HIP(hipHostMalloc(ptr, bytes, hipHostMallocDefault));
+  return NT2H_OK;
+}
+
+int nt2h_host_free(void* ptr) {
+  if (!ptr) return NT2H_OK;
+  NT2H_HIP(hipHostFree(ptr));
+  return NT2H_OK;
+}
+
+int nt2h_memcpy(void* dst, const void* src, size_t bytes, int kind, nt2h_stream_t s) {
+  if (bytes == 0) return NT2H_OK;
+  if (!dst || !src) return NT2H_ERR_NULL_POINTER;
+  hipMemcpyKind k;
+  switch (kind) {
+    case NT2H_H2H: k = hipMemcpyHostToHost; break;
+    case NT2H_H2D: k = hipMemcpyHostToDevice; break;
+    case NT2H_D2H: k = hipMemcpyDeviceToHost; break;
+    case NT2H_D2D: k = hipMemcpyDeviceToDevice; break;
+    case NT2H_DEFAULT: k = hipMemcpyDefault; break;
+    default: return NT2H_ERR_INVALID_ARG;
+  }
+  NT2H_HIP(hipMemcpyAsync(dst, src, bytes, k, as_hip(s)));
+  return NT2H_OK;
+}
+
+int nt2h_sync(nt2h_stream_t s) {
+  NT2H_HIP(hipStreamSynchronize(as_hip(s)));
+  return NT2H_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,132 @@
+"""Torch-facing wrappers over the libnt2hip C ABI.
+
+Tensors must be contiguous CUDA (HIP) tensors of the stated dtype; each call
+launches on torch's current stream.  These mirror NT2's table operations on
+the path: ``Y = a*X + Y`` (saxpy), ``Y = f(X)`` (one functor node),
+``globalsum``/``dot`` (whole-table inner_fold), the Black-Scholes loop nest
+and its adjoint, and the column fold with its adjoint.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+from . import (DEFAULT, EXACT, F64_OPS, FUSED, OPS, SUM_KAHAN, call)
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+def _stream():
+    return C.c_void_p(_torch().cuda.current_stream().cuda_stream)
+
+
+def _ptr(t) -> C.c_void_p:
+    return C.c_void_p(t.data_ptr()) if t is not None else C.c_void_p(0)
+
+
+def _chk(t, dtype, name="tensor"):
+    torch = _torch()
+    if not t.is_cuda:
+        raise ValueError(f"{name} must be a CUDA tensor")
+    if t.dtype != dtype:
+        raise TypeError(f"{name} must be {dtype}, got {t.dtype}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    return t
+
+
+def saxpy_(a: float, x, y):
+    """y <- a*x + y in place (two roundings, bit-identical to NT2's scalar path)."""
+    torch = _torch()
+    _chk(x, torch.float32, "x"), _chk(y, torch.float32, "y")
+    assert x.numel() == y.numel()
+    call("nt2h_saxpy_f32", C.c_float(a), _ptr(x), _ptr(y), y.numel(), _stream())
+    return y
+
+
+def map(op: str, x, precision: int = EXACT, out=None):  # noqa: A001 (NT2 vocabulary)
+    torch = _torch()
+    if x.dtype == torch.float32:
+        fn, code = "nt2h_map_f32", OPS[op]
+    elif x.dtype == torch.float64:
+        fn, code = "nt2h_map_f64", F64_OPS[op]
+    else:
+        raise TypeError("map: float32 or float64")
+    _chk(x, x.dtype, "x")
+    y = torch.empty_like(x) if out is None else _chk(out, x.dtype, "out")
+    call(fn, code, precision, _ptr(x), _ptr(y), x.numel(), _stream())
+    return y
+
+
+def bs_fwd(S, K, T, r, sig, precision: int = EXACT):
+    """Black-Scholes call and put (5 input streams -> 2 output streams, one kernel)."""
+    torch = _torch()
+    dt = S.dtype
+    for n, t in zip("SKTrs", (S, K, T, r, sig)):
+        _chk(t, dt, n)
+    call_, put = torch.empty_like(S), torch.empty_like(S)
+    fn = "nt2h_bs_fwd_f32" if dt == torch.float32 else "nt2h_bs_fwd_f64"
+    call(fn, precision, *[_ptr(t) for t in (S, K, T, r, sig, call_, put)], S.numel(), _stream())
+    return call_, put
+
+
+def _fold(fn, tensors, n, extra, out=None):
+    torch = _torch()
+    res = torch.empty(1, dtype=torch.float64, device=tensors[0].device) if out is None else out
+    call(fn, *[_ptr(t) for t in tensors], n, *extra, _ptr(res), _stream())
+    return res
+
+
+def sum(x, mode: int = SUM_KAHAN, out=None):  # noqa: A001 (NT2 vocabulary: sum / globalsum)
+    """Whole-table sum; returns a 1-element float64 device tensor."""
+    torch = _torch()
+    if x.dtype == torch.float32:
+        return _fold("nt2h_sum_f32", [_chk(x, torch.float32, "x")], x.numel(), [mode], out)
+    _chk(x, torch.float64, "x")
+    return _fold("nt2h_sum_f64", [x], x.numel(), [], out)
+
+
+def dot(x, y, mode: int = SUM_KAHAN, out=None):
+    torch = _torch()
+    assert x.numel() == y.numel()
+    if x.dtype == torch.float32:
+        return _fold("nt2h_dot_f32", [_chk(x, torch.float32, "x"), _chk(y, torch.float32, "y")], x.numel(),
+                     [mode], out)
+    return _fold("nt2h_dot_f64", [_chk(x, torch.float64, "x"), _chk(y, torch.float64, "y")], x.numel(), [], out)
+
+
+def bs_fwd_adj(S, K, T, r, sig, w, totals: bool = True, outs=None):
+    """price and the w-weighted input cotangents (dS, dK, dT, dr, dsig), plus
+    totals = [sum dr, sum dsig] as a float64 device tensor (or None)."""
+    torch = _torch()
+    dt = S.dtype
+    for n, t in zip(("S", "K", "T", "r", "sig", "w"), (S, K, T, r, sig, w)):
+        _chk(t, dt, n)
+    if outs is None:
+        outs = [torch.empty_like(S) for _ in range(6)]
+    tot = torch.zeros(2, dtype=torch.float64, device=S.device) if totals else None
+    fn = "nt2h_bs_fwd_adj_f32" if dt == torch.float32 else "nt2h_bs_fwd_adj_f64"
+    call(fn, *[_ptr(t) for t in (S, K, T, r, sig, w)], *[_ptr(o) for o in outs], _ptr(tot), S.numel(), _stream())
+    return (*outs, tot)
+
+
+def colfold_gc(X, s, k, p, gz=None, grad: bool = True):
+    """X: (cols, rows) row-major CUDA tensor == column-major (rows, cols) table.
+
+    Returns z and, when grad, (dz_ds, dz_dk, dz_dp) for cotangent gz (None = ones)."""
+    torch = _torch()
+    _chk(X, torch.float32, "X")
+    cols, rows = X.shape
+    for n, t in zip("skp", (s, k, p)):
+        _chk(t, torch.float32, n)
+    z = torch.empty(cols, dtype=torch.float32, device=X.device)
+    g = [torch.empty_like(z) for _ in range(3)] if grad else [None, None, None]
+    call("nt2h_colfold_gc_f32", _ptr(X), rows, cols, rows, _ptr(s), _ptr(k), _ptr(p), _ptr(gz), _ptr(z),
+         *[_ptr(t) for t in g], _stream())
+    return (z, *g) if grad else z
+
+
+__all__ = ["saxpy_", "map", "bs_fwd", "sum", "dot", "bs_fwd_adj", "colfold_gc", "EXACT", "FUSED", "DEFAULT"]

@@ -1,0 +1,362 @@
+"""GPU parity: every libnt2hip entry point against the oracle on the same inputs.
+
+Bars (stated per test): bit-exact where the GPU evaluates NT2's scalar
+arithmetic (saxpy, EXACT maps, EXACT Black-Scholes), a stated std-ULP bound
+for FUSED arithmetic, and relative/scale-normalised error bounds against
+the FP64 oracle for the compensated folds and the adjoints.
+"""
+import numpy as np
+import pytest
+
+import nt2_amd
+from nt2_amd import ops, synth
+
+pytestmark = pytest.mark.gpu
+
+F32_OPS = ["exp", "log", "sqrt", "erf", "erfc", "normcdf", "tanh", "sin", "cos", "normpdf", "fastnormcdf",
+           "exp2", "exp10"]
+F64_OPS = ["exp", "log", "sqrt", "erf", "erfc", "normcdf", "tanh", "sin", "cos", "normpdf"]
+
+# argument ranges per functor for the 2^20-sample parity sweep
+RANGES = {
+    "exp": (-100, 100), "exp2": (-140, 140), "exp10": (-40, 40), "log": None, "sqrt": None,
+    "erf": (-8, 8), "erfc": (-20, 20), "normcdf": (-30, 30), "normpdf": (-30, 30), "fastnormcdf": (-30, 30),
+    "tanh": (-20, 20), "sin": (-1e6, 1e6), "cos": (-1e6, 1e6),
+}
+SPECIALS = [0.0, -0.0, 1.0, -1.0, 0.5, np.inf, -np.inf, np.nan, 1e-45, -1e-45, 1e-38, 88.3762626647949,
+            -88.3762626647949, 88.72, -87.7, -87.3, 0.6666666, 0.625, 0.785398163, 1.57079632679, 3.14159265,
+            62.83185, 201.06, 823549.6, 823549.7, 1e6, 3e7, 3.4e38, -3.4e38, 9.3, 10.05, 127.0, -127.0, 38.2, -37.9]
+
+# FUSED arithmetic: max std-ULP distance to the NT2 scalar result, per functor
+# (erfc's own error is up to 16 ulp on [2/3, inf): FMA moves it by as much)
+FUSED_ULP = {"exp": 2, "exp2": 2, "exp10": 2, "log": 2, "sqrt": 0, "erf": 4, "erfc": 16, "normcdf": 16,
+             "normpdf": 4, "fastnormcdf": 8, "tanh": 4, "sin": 4, "cos": 4}
+
+
+def _args(fn, n, dtype=np.float32):
+    rng = np.random.default_rng(1234)
+    r = RANGES[fn]
+    if r is None:
+        x = 10.0 ** rng.uniform(-45, 38.5, n)
+    else:
+        x = rng.uniform(r[0], r[1], n)
+    x = np.concatenate([np.array(SPECIALS), x]).astype(dtype)
+    return x
+
+
+def _gpu(x, dev):
+    import torch
+
+    return torch.from_numpy(np.ascontiguousarray(x)).to(dev)
+
+
+def _bits_equal(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    same = (a.view(np.uint32 if a.dtype == np.float32 else np.uint64) ==
+            b.view(np.uint32 if b.dtype == np.float32 else np.uint64))
+    both_nan = np.isnan(a) & np.isnan(b)
+    return same | both_nan
+
+
+def test_saxpy_bit_exact(dev, ref):
+    x, y = synth.saxpy_inputs(1 << 20)
+    gy = _gpu(y, dev)
+    ops.saxpy_(2.5, _gpu(x, dev), gy)
+    got = gy.cpu().numpy()
+    want = ref.saxpy_f32(2.5, x, y)
+    assert _bits_equal(got, want).all()
+
+
+def test_saxpy_known_answer_and_ragged(dev):
+    # demo/saxpy: X(i) = Y(i) = i-1 -> (alpha+1)(i-1) exactly
+    for n in (1, 3, 5, 1023, 4097):
+        v = np.arange(n, dtype=np.float32)
+        gy = _gpu(v.copy(), dev)
+        ops.saxpy_(2.5, _gpu(v, dev), gy)
+        assert np.array_equal(gy.cpu().numpy(), (3.5 * v).astype(np.float32))
+
+
+def test_saxpy_unaligned(dev, ref):
+    import torch
+
+    x, y = synth.saxpy_inputs(4099)
+    gx = _gpu(np.concatenate([[0], x]).astype(np.float32), dev)[1:]
+    gy = _gpu(np.concatenate([[0], y]).astype(np.float32), dev)[1:]
+    assert gx.data_ptr() % 16 != 0
+    ops.saxpy_(2.5, gx, gy)
+    torch.cuda.synchronize()
+    assert _bits_equal(gy.cpu().numpy(), ref.saxpy_f32(2.5, x, y)).all()
+
+
+@pytest.mark.parametrize("fn", F32_OPS)
+def test_map_f32_exact_bit_identical(dev, ref, fn):
+    x = _args(fn, 1 << 20)
+    got = ops.map(fn, _gpu(x, dev), nt2_amd.EXACT).cpu().numpy()
+    want = ref.map_f32(fn, x)
+    eq = _bits_equal(got, want)
+    if fn in ("sin", "cos"):
+        # beyond 2^18 pi NT2 reduces with fdlibm's float Payne-Hanek, the GPU
+        # with an integer Payne-Hanek giving the correctly rounded argument:
+        # allowed to differ by 2 std-ULP there, bit-exact everywhere else
+        big = np.abs(x) > np.float32(823549.6875)
+        assert eq[~big].all(), x[~big][~eq[~big]][:8]
+        st = ref.cmp_f32(got[big], want[big])
+        assert st["max_std_ulp"] <= 2 and st["nan_mismatch"] == 0
+    else:
+        assert eq.all(), (x[~eq][:8], got[~eq][:8], want[~eq][:8])
+
+
+@pytest.mark.parametrize("fn", F32_OPS)
+def test_map_f32_fused_bound(dev, ref, fn):
+    x = _args(fn, 1 << 20)
+    got = ops.map(fn, _gpu(x, dev), nt2_amd.FUSED).cpu().numpy()
+    want = ref.map_f32(fn, x)
+    st = ref.cmp_f32(got, want)
+    assert st["nan_mismatch"] == 0
+    if fn in ("erfc", "normcdf", "fastnormcdf", "normpdf", "exp", "exp2", "exp10", "erf"):
+        # near exp's flush-to-zero edge a 1-ulp change of the argument moves
+        # the result to/from 0: compare above the smallest normal only
+        m = (np.abs(want) >= np.float32(1.1754944e-38)) & (np.abs(got) >= np.float32(1.1754944e-38))
+        st = ref.cmp_f32(got[m], want[m])
+    assert st["max_std_ulp"] <= FUSED_ULP[fn], st
+
+
+@pytest.mark.parametrize("fn", F64_OPS)
+def test_map_f64_exact(dev, ref, fn):
+    x = _args(fn, 1 << 18, np.float64)
+    got = ops.map(fn, _gpu(x, dev), nt2_amd.EXACT).cpu().numpy()
+    want = ref.map_f64(fn, x)
+    eq = _bits_equal(got, want)
+    if fn in ("sin", "cos"):
+        big = np.abs(x) > 1647099.3291652855  # 2^20 pi/2: Payne-Hanek on both sides
+        assert eq[~big].all(), x[~big][~eq[~big]][:8]
+        st = ref.ulp_stats_f64(got[big], want[big])
+        assert st["max_ulp"] <= 2
+    else:
+        assert eq.all(), (x[~eq][:8], got[~eq][:8], want[~eq][:8])
+
+
+def test_map_in_place_and_empty(dev, ref):
+    import torch
+
+    x = _args("exp", 4096)
+    g = _gpu(x, dev)
+    ops.map("exp", g, nt2_amd.EXACT, out=g)
+    assert _bits_equal(g.cpu().numpy(), ref.map_f32("exp", x)).all()
+    e = torch.empty(0, device=dev)
+    assert ops.map("exp", e).numel() == 0
+
+
+def _ulp_f32_vs(v, exact):
+    f = np.float32(exact)
+    sp = np.spacing(np.abs(f))
+    return abs(v - exact) / sp
+
+
+@pytest.mark.parametrize("mode", [nt2_amd.SUM_NAIVE, nt2_amd.SUM_KAHAN, nt2_amd.SUM_PAIRWISE])
+def test_sdot_config1(dev, ref, mode):
+    x, y = synth.saxpy_inputs(1 << 20)
+    got = ops.dot(_gpu(x, dev), _gpu(y, dev), mode).item()
+    hi, lo = ref.exact_dot_f32(x, y)
+    exact = hi + lo
+    u = _ulp_f32_vs(float(np.float32(got)), exact)
+    bound = {nt2_amd.SUM_KAHAN: 1, nt2_amd.SUM_PAIRWISE: 4, nt2_amd.SUM_NAIVE: 16}[mode]
+    assert u <= bound, (got, exact, u)
+    if mode == nt2_amd.SUM_KAHAN:
+        # the KAHAN dot is within 4 ulp of the correctly rounded dot (the
+        # config-1 gate); NT2's sequential float order is far from both
+        seq = float(ref.fold_dot_f32(x, y))
+        print("sdot: gpu %.9g exact %.12g nt2-sequential %.9g (%.1f ulp from exact)" %
+              (got, exact, seq, _ulp_f32_vs(seq, exact)))
+
+
+@pytest.mark.parametrize("n", [0, 1, 3, 4, 5, 255, 1023, 4097, 1 << 16])
+def test_sum_ragged_sizes(dev, ref, n):
+    a, _ = synth.sumdot_inputs(max(n, 1))
+    a = a[:n]
+    got = ops.sum(_gpu(a, dev), nt2_amd.SUM_KAHAN).item()
+    hi, lo = ref.exact_sum_f32(a)
+    assert abs(got - (hi + lo)) <= 1e-12 * max(1.0, abs(hi))
+
+
+def test_sum_ill_conditioned_kahan(dev, ref):
+    n = 1 << 24
+    a, b = synth.sumdot_inputs(n)
+    ga, gb = _gpu(a, dev), _gpu(b, dev)
+    hi, lo = ref.exact_sum_f32(a)
+    exact = hi + lo
+    k = ops.sum(ga, nt2_amd.SUM_KAHAN).item()
+    p = ops.sum(ga, nt2_amd.SUM_PAIRWISE).item()
+    nv = ops.sum(ga, nt2_amd.SUM_NAIVE).item()
+    assert _ulp_f32_vs(float(np.float32(k)), exact) <= 1
+    assert abs(p - exact) / exact < 1e-6
+    assert abs(nv - exact) / exact < 1e-5
+    seq = float(ref.fold_sum_f32(a))
+    assert abs(seq - exact) / exact > abs(k - exact) / exact  # NT2 sequential float is worse
+    dh, dl = ref.exact_dot_f32(a, b)
+    d = ops.dot(ga, gb, nt2_amd.SUM_KAHAN).item()
+    assert _ulp_f32_vs(float(np.float32(d)), dh + dl) <= 1
+    # determinism: same bits on a second run
+    assert ops.sum(ga, nt2_amd.SUM_KAHAN).item() == k
+
+
+def test_sum_host_output_pointer(dev, ref):
+    import ctypes as C
+
+    a, _ = synth.sumdot_inputs(100000)
+    g = _gpu(a, dev)
+    out = (C.c_double * 1)()
+    import torch
+
+    nt2_amd.call("nt2h_sum_f32", C.c_void_p(g.data_ptr()), g.numel(), nt2_amd.SUM_KAHAN, C.cast(out, C.c_void_p),
+                 C.c_void_p(torch.cuda.current_stream().cuda_stream))
+    hi, lo = ref.exact_sum_f32(a)
+    assert abs(out[0] - (hi + lo)) <= 1e-12 * abs(hi)
+
+
+def test_sum_dot_f64(dev, ref):
+    a, b = synth.sumdot_inputs(1 << 20)
+    a64, b64 = a.astype(np.float64), b.astype(np.float64)
+    s = ops.sum(_gpu(a64, dev)).item()
+    d = ops.dot(_gpu(a64, dev), _gpu(b64, dev)).item()
+    hi, lo = ref.exact_sum_f32(a)
+    dh, dl = ref.exact_dot_f32(a, b)
+    assert abs(s - (hi + lo)) / (hi + lo) < 1e-13
+    assert abs(d - (dh + dl)) / (dh + dl) < 1e-13
+
+
+def test_bs_fwd_f32_exact_bit_identical(dev, ref):
+    d = synth.bs_inputs(1 << 20)
+    g = [_gpu(d[k], dev) for k in ("S", "K", "T", "r", "sig")]
+    c, p = ops.bs_fwd(*g, precision=nt2_amd.EXACT)
+    wc, wp = ref.bs_fwd(d["S"], d["K"], d["T"], d["r"], d["sig"])
+    assert _bits_equal(c.cpu().numpy(), wc).all()
+    assert _bits_equal(p.cpu().numpy(), wp).all()
+
+
+def test_bs_fwd_f64_exact_bit_identical(dev, ref):
+    d = synth.bs_inputs(1 << 18)
+    d = {k: v.astype(np.float64) for k, v in d.items()}
+    g = [_gpu(d[k], dev) for k in ("S", "K", "T", "r", "sig")]
+    c, p = ops.bs_fwd(*g, precision=nt2_amd.EXACT)
+    wc, wp = ref.bs_fwd(d["S"], d["K"], d["T"], d["r"], d["sig"], dtype=np.float64)
+    assert _bits_equal(c.cpu().numpy(), wc).all()
+    assert _bits_equal(p.cpu().numpy(), wp).all()
+
+
+def _scaled_err(got, want, S):
+    return np.max(np.abs(got.astype(np.float64) - want) / np.maximum(np.abs(want), 1e-6 * S))
+
+
+def test_bs_fwd_f32_fused_vs_fp64(dev, ref):
+    d = synth.bs_inputs(1 << 20)
+    g = [_gpu(d[k], dev) for k in ("S", "K", "T", "r", "sig")]
+    c, p = ops.bs_fwd(*g, precision=nt2_amd.FUSED)
+    d64 = {k: v.astype(np.float64) for k, v in d.items()}
+    wc, wp = ref.bs_fwd(d64["S"], d64["K"], d64["T"], d64["r"], d64["sig"], dtype=np.float64)
+    S = d64["S"]
+    assert _scaled_err(c.cpu().numpy(), wc, S) < 2e-4
+    assert _scaled_err(p.cpu().numpy(), wp, S) < 2e-4
+    # put-call parity on the GPU results: call - put = S - K e^{-rT}
+    par = c.cpu().numpy().astype(np.float64) - p.cpu().numpy() - (S - d64["K"] * np.exp(-d64["r"] * d64["T"]))
+    assert np.max(np.abs(par) / np.maximum(S, d64["K"])) < 1e-6
+
+
+def test_bs_fwd_adj_f32_vs_fp64_oracle(dev, ref):
+    n = (1 << 20) + 3  # ragged tail through the scalar loop
+    d = synth.bs_inputs(n, with_w=True)
+    keys = ("S", "K", "T", "r", "sig", "w")
+    g = [_gpu(d[k], dev) for k in keys]
+    outs = ops.bs_fwd_adj(*g)
+    got = [o.cpu().numpy() for o in outs[:6]]
+    tot = outs[6].cpu().numpy()
+    d64 = {k: d[k].astype(np.float64) for k in keys}
+    want = ref.bs_fwd_adj(*[d64[k] for k in keys], dtype=np.float64)
+    S = d64["S"]
+    w = np.abs(d64["w"])
+    # price: |err| / max(|p|, 1e-6 S); gradients: relative to |w| x the scale of each greek
+    assert _scaled_err(got[0], want[0], S) < 2e-4
+    scales = [None, w, w * np.exp(-d64["r"] * d64["T"]), w * S, w * S * d64["T"], w * S * np.sqrt(d64["T"])]
+    for i in range(1, 6):
+        err = np.abs(got[i].astype(np.float64) - want[i]) / np.maximum(scales[i], 1e-30)
+        assert np.max(err) < 1e-5, (i, np.max(err))
+    # totals: compensated sum of the float gradients vs the FP64 oracle total
+    tr = want[6]
+    assert abs(tot[0] - tr[0]) <= 1e-5 * np.sum(np.abs(want[4]))
+    assert abs(tot[1] - tr[1]) <= 1e-5 * np.sum(np.abs(want[5]))
+    # and exactly the sum of the float gradient streams it wrote
+    assert abs(tot[0] - got[4].astype(np.float64).sum()) <= 1e-9 * np.sum(np.abs(got[4]))
+
+
+def test_bs_fwd_adj_f64(dev, ref):
+    n = 1 << 18
+    d = synth.bs_inputs(n, with_w=True)
+    keys = ("S", "K", "T", "r", "sig", "w")
+    d64 = {k: d[k].astype(np.float64) for k in keys}
+    outs = ops.bs_fwd_adj(*[_gpu(d64[k], dev) for k in keys])
+    want = ref.bs_fwd_adj(*[d64[k] for k in keys], dtype=np.float64)
+    S = d64["S"]
+    assert _scaled_err(outs[0].cpu().numpy(), want[0], S) < 1e-10
+    for i in range(1, 6):
+        g = outs[i].cpu().numpy()
+        assert np.max(np.abs(g - want[i]) / np.maximum(np.abs(want[i]), 1e-300 + 1e-12 * S)) < 1e-9
+
+
+def test_bs_adjoint_vs_central_differences(dev, ref):
+    """GPU float adjoint vs central differences of the FP64 oracle forward (h = 1e-3 relative)."""
+    n = 4096
+    d = synth.bs_inputs(n, with_w=True)
+    keys = ("S", "K", "T", "r", "sig", "w")
+    outs = ops.bs_fwd_adj(*[_gpu(d[k], dev) for k in keys])
+    got = [o.cpu().numpy().astype(np.float64) for o in outs[1:6]]
+    d64 = {k: d[k].astype(np.float64) for k in keys}
+    for j, k in enumerate(("S", "K", "T", "r", "sig")):
+        h = 1e-3 * np.abs(d64[k])
+        up = dict(d64); up[k] = d64[k] + h
+        dn = dict(d64); dn[k] = d64[k] - h
+        cu, _ = ref.bs_fwd(*[up[q] for q in ("S", "K", "T", "r", "sig")], dtype=np.float64)
+        cd, _ = ref.bs_fwd(*[dn[q] for q in ("S", "K", "T", "r", "sig")], dtype=np.float64)
+        fd = d64["w"] * (cu - cd) / (2 * h)
+        scale = np.abs(d64["w"]) * np.maximum(np.abs(fd / np.maximum(np.abs(d64["w"]), 1e-30)), 1e-3 * d64["S"])
+        err = np.abs(got[j] - fd) / scale
+        assert np.max(err) < 2e-3, (k, np.max(err))
+
+
+def test_colfold_small_vs_oracle(dev, ref):
+    rows, cols = 1000, 64  # rows not a multiple of 4: scalar tail per column
+    X, s, k, p = synth.colfold_inputs(rows, cols)
+    gz = synth.uniform(0, 77, cols, -1.0, 1.0)
+    z, ds, dk, dp = ops.colfold_gc(_gpu(X, dev), _gpu(s, dev), _gpu(k, dev), _gpu(p, dev), gz=_gpu(gz, dev))
+    wz, wds, wdk, wdp = ref.colfold_gc_f64(X.T, s, k, p, gz)
+    absX = np.abs(X.astype(np.float64))
+    e = np.exp(-(X.astype(np.float64) ** 2) / (2 * s[:, None].astype(np.float64) ** 2))
+    mag = e.sum(axis=1)  # sum of |terms| scale per column
+    assert np.max(np.abs(z.cpu().numpy() - wz) / mag) < 1e-6
+    assert np.max(np.abs(dp.cpu().numpy() - wdp) / (np.abs(gz) * mag + 1e-30)) < 1e-6
+    magk = (e * absX).sum(axis=1)
+    assert np.max(np.abs(dk.cpu().numpy() - wdk) / (np.abs(gz) * magk + 1e-30)) < 1e-6
+    mags = (e * absX ** 2).sum(axis=1) / s.astype(np.float64) ** 3
+    assert np.max(np.abs(ds.cpu().numpy() - wds) / (np.abs(gz) * mags + 1e-30)) < 1e-6
+    # forward-only launch gives the same z
+    z2 = ops.colfold_gc(_gpu(X, dev), _gpu(s, dev), _gpu(k, dev), _gpu(p, dev), grad=False)
+    assert np.max(np.abs(z2.cpu().numpy() - wz) / mag) < 1e-6
+
+
+def test_colfold_config5_shape(dev, ref):
+    """16384 x 8192 column fold: GPU vs FP64 oracle on a column subset, full-shape properties."""
+    rows, cols = 16384, 8192
+    X, s, k, p = synth.colfold_inputs(rows, cols)
+    gX = _gpu(X, dev)
+    z, ds, dk, dp = ops.colfold_gc(gX, _gpu(s, dev), _gpu(k, dev), _gpu(p, dev))
+    z, ds, dk, dp = (t.cpu().numpy() for t in (z, ds, dk, dp))
+    assert np.isfinite(z).all() and np.isfinite(ds).all() and np.isfinite(dk).all() and np.isfinite(dp).all()
+    sel = np.arange(0, cols, 97)
+    wz, wds, wdk, wdp = ref.colfold_gc_f64(X[sel].T, s[sel], k[sel], p[sel])
+    e = np.exp(-(X[sel].astype(np.float64) ** 2) / (2 * s[sel, None].astype(np.float64) ** 2))
+    mag = e.sum(axis=1)
+    assert np.max(np.abs(z[sel] - wz) / mag) < 1e-6
+    assert np.max(np.abs(dp[sel] - wdp) / mag) < 1e-6
+    # determinism of the fold
+    z2 = ops.colfold_gc(gX, _gpu(s, dev), _gpu(k, dev), _gpu(p, dev), grad=False).cpu().numpy()
+    assert np.array_equal(z, z2)
