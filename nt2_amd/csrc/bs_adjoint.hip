@@ -106,8 +106,8 @@ struct BSAdjArgsD {
 };
 
 template <bool VEC, bool TOTALS>
-__global__ __launch_bounds__(kBlock) void bs_fwd_adj_f32_kernel(BSAdjArgs a, int64_t n, double* partials,
-                                                                unsigned* counter, double* totals) {
+__global__ __launch_bounds__(kBlock) void bs_fwd_adj_f32_kernel(BSAdjArgs a, int64_t n, Workspace ws,
+                                                                double* totals) {
   Comp acc[2] = {{0.0f, 0.0f}, {0.0f, 0.0f}};
   const int64_t stride = (int64_t)gridDim.x * kBlock;
   const int64_t gid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -156,12 +156,8 @@ __global__ __launch_bounds__(kBlock) void bs_fwd_adj_f32_kernel(BSAdjArgs a, int
     __shared__ float lds[2 * 2 * kWaves];
     double bsum[2];
     block_reduce_comp<2>(acc, lds, bsum);
-    if (threadIdx.x == 0) {
-      partials[blockIdx.x] = bsum[0];
-      partials[gridDim.x + blockIdx.x] = bsum[1];
-    }
     double res[2];
-    if (grid_combine<2>(partials, counter, res) && threadIdx.x == 0) {
+    if (grid_combine<2>(bsum, ws, res)) {
       totals[0] = res[0];
       totals[1] = res[1];
     }
@@ -169,8 +165,8 @@ __global__ __launch_bounds__(kBlock) void bs_fwd_adj_f32_kernel(BSAdjArgs a, int
 }
 
 template <bool TOTALS>
-__global__ __launch_bounds__(kBlock) void bs_fwd_adj_f64_kernel(BSAdjArgsD a, int64_t n, double* partials,
-                                                                unsigned* counter, double* totals) {
+__global__ __launch_bounds__(kBlock) void bs_fwd_adj_f64_kernel(BSAdjArgsD a, int64_t n, Workspace ws,
+                                                                double* totals) {
   double acc[2] = {0.0, 0.0};
   const int64_t stride = (int64_t)gridDim.x * kBlock;
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
@@ -197,29 +193,22 @@ __global__ __launch_bounds__(kBlock) void bs_fwd_adj_f64_kernel(BSAdjArgsD a, in
       lds[kWaves + (threadIdx.x >> 6)] = acc[1];
     }
     __syncthreads();
+    double bsum[2] = {0.0, 0.0};
     if (threadIdx.x == 0) {
       double t0 = lds[0], t1 = lds[kWaves];
       for (int w = 1; w < kWaves; ++w) {
         t0 += lds[w];
         t1 += lds[kWaves + w];
       }
-      partials[blockIdx.x] = t0;
-      partials[gridDim.x + blockIdx.x] = t1;
+      bsum[0] = t0;
+      bsum[1] = t1;
     }
     double res[2];
-    if (grid_combine<2>(partials, counter, res) && threadIdx.x == 0) {
+    if (grid_combine<2>(bsum, ws, res)) {
       totals[0] = res[0];
       totals[1] = res[1];
     }
   }
-}
-
-static unsigned adj_grid(int64_t n, int vec) {
-  int64_t units = (n + vec - 1) / vec;
-  int64_t b = (units + kBlock - 1) / kBlock;
-  if (b < 1) b = 1;
-  if (b > kMaxReduceBlocks) b = kMaxReduceBlocks;
-  return (unsigned)b;
 }
 
 }  // namespace nt2h
@@ -252,14 +241,14 @@ int nt2h_bs_fwd_adj_f32(const float* S, const float* K, const float* T, const fl
   }
   BSAdjArgs a{{S, K, T, r, sig, w}, {price, dS, dK, dT, dr, dsig}};
   bool vec = aligned16(S, K, T, r, sig, w, price, dS, dK, dT, dr, dsig);
-  unsigned g = adj_grid(n, vec ? 4 : 1);
+  unsigned g = fold_blocks(vec ? (n + 3) / 4 : n);
   hipStream_t st = as_hip(s);
   if (vec) {
-    if (totals) hipLaunchKernelGGL((bs_fwd_adj_f32_kernel<true, true>), dim3(g), dim3(kBlock), 0, st, a, n, ws.partials, ws.counter, dst);
-    else hipLaunchKernelGGL((bs_fwd_adj_f32_kernel<true, false>), dim3(g), dim3(kBlock), 0, st, a, n, ws.partials, ws.counter, dst);
+    if (totals) hipLaunchKernelGGL((bs_fwd_adj_f32_kernel<true, true>), dim3(g), dim3(kBlock), 0, st, a, n, ws, dst);
+    else hipLaunchKernelGGL((bs_fwd_adj_f32_kernel<true, false>), dim3(g), dim3(kBlock), 0, st, a, n, ws, dst);
   } else {
-    if (totals) hipLaunchKernelGGL((bs_fwd_adj_f32_kernel<false, true>), dim3(g), dim3(kBlock), 0, st, a, n, ws.partials, ws.counter, dst);
-    else hipLaunchKernelGGL((bs_fwd_adj_f32_kernel<false, false>), dim3(g), dim3(kBlock), 0, st, a, n, ws.partials, ws.counter, dst);
+    if (totals) hipLaunchKernelGGL((bs_fwd_adj_f32_kernel<false, true>), dim3(g), dim3(kBlock), 0, st, a, n, ws, dst);
+    else hipLaunchKernelGGL((bs_fwd_adj_f32_kernel<false, false>), dim3(g), dim3(kBlock), 0, st, a, n, ws, dst);
   }
   NT2H_HIP(hipGetLastError());
   return totals ? deliver(s, totals, ws.result, 2, host) : NT2H_OK;
@@ -288,9 +277,9 @@ int nt2h_bs_fwd_adj_f64(const double* S, const double* K, const double* T, const
     return NT2H_OK;
   }
   BSAdjArgsD a{{S, K, T, r, sig, w}, {price, dS, dK, dT, dr, dsig}};
-  unsigned g = adj_grid(n, 1);
-  if (totals) hipLaunchKernelGGL((bs_fwd_adj_f64_kernel<true>), dim3(g), dim3(kBlock), 0, as_hip(s), a, n, ws.partials, ws.counter, dst);
-  else hipLaunchKernelGGL((bs_fwd_adj_f64_kernel<false>), dim3(g), dim3(kBlock), 0, as_hip(s), a, n, ws.partials, ws.counter, dst);
+  unsigned g = fold_blocks(n);
+  if (totals) hipLaunchKernelGGL((bs_fwd_adj_f64_kernel<true>), dim3(g), dim3(kBlock), 0, as_hip(s), a, n, ws, dst);
+  else hipLaunchKernelGGL((bs_fwd_adj_f64_kernel<false>), dim3(g), dim3(kBlock), 0, as_hip(s), a, n, ws, dst);
   NT2H_HIP(hipGetLastError());
   return totals ? deliver(s, totals, ws.result, 2, host) : NT2H_OK;
 }

@@ -15,7 +15,9 @@ typedef double f64x2 __attribute__((ext_vector_type(2)));
 
 constexpr int kBlock = 256;          // 4 wave64 per workgroup
 constexpr int kWaves = kBlock / 64;
-constexpr int kMaxReduceBlocks = 2048;  // 8 per CU on 256 CUs
+constexpr int kMaxFoldBlocks = 65536;  // workgroups of one fold launch (partials slots)
+constexpr int kGroup = 64;            // workgroups combined per first-level ticket
+constexpr int kMaxGroups = kMaxFoldBlocks / kGroup;
 constexpr int kMaxPartials = 4;       // accumulators per block in one fold
 
 // ---------------------------------------------------------------- host
@@ -51,7 +53,7 @@ template <class... Ps> inline bool aligned16(Ps... ps) {
 
 // Grid for a streaming kernel over n elements handled vec at a time with a
 // grid-stride loop: enough workgroups to fill 256 CUs several times over.
-inline unsigned stream_grid(int64_t n, int vec, int64_t max_blocks = 8192) {
+inline unsigned stream_grid(int64_t n, int vec, int64_t max_blocks = 65536) {
   int64_t units = (n + vec - 1) / vec;
   int64_t b = (units + kBlock - 1) / kBlock;
   if (b < 1) b = 1;
@@ -59,14 +61,23 @@ inline unsigned stream_grid(int64_t n, int vec, int64_t max_blocks = 8192) {
   return (unsigned)b;
 }
 
-// Reduction workspace of one (device, stream): kMaxReduceBlocks *
-// kMaxPartials doubles of per-block partials, a completion counter that the
-// last block resets, and a result slot for host-pointer outputs.
+// Reduction workspace of one (device, stream): per-block partials
+// (kMaxPartials x kMaxFoldBlocks doubles), per-group partials (kMaxPartials x
+// kMaxGroups), kMaxGroups + 1 ticket counters that the kernels reset after
+// use, and a result slot for host-pointer outputs.
 struct Workspace {
   double* partials;
-  unsigned* counter;
+  double* gpartials;
+  unsigned* counters;
   double* result;  // kMaxPartials doubles
 };
+// Workgroups for a fold over `units` vectors, one vector per lane per step.
+inline unsigned fold_blocks(int64_t units) {
+  int64_t b = (units + kBlock - 1) / kBlock;
+  if (b < 1) b = 1;
+  if (b > kMaxFoldBlocks) b = kMaxFoldBlocks;
+  return (unsigned)b;
+}
 int get_workspace(nt2h_stream_t s, Workspace* ws);
 // Deliver nv doubles computed into dev (device memory) to out (device or
 // host memory); for host memory the stream is synchronised.
@@ -131,52 +142,82 @@ __device__ __forceinline__ void block_reduce_comp(Comp (&acc)[NV], float* lds, d
   }
 }
 
-// Last-block combine of per-block partials (gridDim.x blocks, NV values
-// each, partials[v * gridDim.x + b]) in FP64, in block order per thread
-// stride and a fixed shuffle tree: deterministic for a given grid.  Call
-// from every thread of the block after the block's own partial is written.
-// Returns true in the one block that performed the combine (results in
-// res[0..NV) written by thread 0); resets the counter for the next launch.
+// Cross-block handoff without an agent-scope fence.  On gfx950 a release
+// fence at agent scope writes back the whole L2 of the XCD, which costs
+// more than the fold itself when every block issues it.  Instead each
+// block publishes its partials with agent-scope atomic stores (written
+// through to the point of coherence of all XCDs), waits for them to
+// complete, and takes a ticket with an agent-scope atomic add; the block
+// holding the last ticket reads the partials back with agent-scope atomic
+// loads (which bypass any stale L2 copy).
+__device__ __forceinline__ void publish(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double fetch(const double* p) {
+  return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Two-level deterministic combine of per-block partials, after
+// block_reduce_comp (or an equivalent) left the block's NV sums in thread 0.
+// Only wave 0 takes part; the other waves leave at once.  Thread 0
+// publishes the block's partials and takes a ticket on its group of kGroup
+// blocks; the block holding the group's last ticket sums the group's
+// partials in block order (one lane per block, fixed shuffle tree), publishes
+// the group partial and takes a ticket on the top counter; the last group
+// sums the group partials in group order.  Every ticket counter is reset by
+// the block that consumed it.  Returns true in lane 0 of the final block,
+// with the totals in res[0..NV).
 template <int NV>
-__device__ __forceinline__ bool grid_combine(double* partials, unsigned* counter, double* res) {
-  __shared__ bool am_last;
-  __shared__ double lds[NV * kWaves];
-  __threadfence();
-  if (threadIdx.x == 0) {
-    unsigned t = atomicAdd(counter, 1u);
-    am_last = (t == gridDim.x - 1);
+__device__ __forceinline__ bool grid_combine(const double (&bsum)[NV], const Workspace& ws, double* res) {
+  if (threadIdx.x >= 64) return false;
+  const unsigned nb = gridDim.x, b = blockIdx.x;
+  const unsigned ng = (nb + kGroup - 1) / kGroup, g = b / kGroup;
+  const unsigned gsize = (nb - g * kGroup) < (unsigned)kGroup ? (nb - g * kGroup) : (unsigned)kGroup;
+  const int lane = threadIdx.x;
+  unsigned t = 0;
+  if (lane == 0) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) publish(&ws.partials[v * nb + b], bsum[v]);
+    __builtin_amdgcn_s_waitcnt(0);
+    t = __hip_atomic_fetch_add(&ws.counters[g], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
-  __syncthreads();
-  if (!am_last) return false;
-  __threadfence();
+  t = __shfl(t, 0, 64);
+  if (t != gsize - 1) return false;
   double acc[NV];
 #pragma unroll
-  for (int v = 0; v < NV; ++v) acc[v] = 0.0;
-  for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
+  for (int v = 0; v < NV; ++v) acc[v] = ((unsigned)lane < gsize) ? fetch(&ws.partials[v * nb + g * kGroup + lane]) : 0.0;
 #pragma unroll
-    for (int v = 0; v < NV; ++v) acc[v] += __builtin_nontemporal_load(&partials[v * gridDim.x + b]);
-  }
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) {
+  for (int m = 1; m < 64; m <<= 1) {
 #pragma unroll
     for (int v = 0; v < NV; ++v) acc[v] += shfl_xor_d(acc[v], m);
   }
-  int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  unsigned t2 = 0;
   if (lane == 0) {
 #pragma unroll
-    for (int v = 0; v < NV; ++v) lds[v * kWaves + wave] = acc[v];
+    for (int v = 0; v < NV; ++v) publish(&ws.gpartials[v * ng + g], acc[v]);
+    __hip_atomic_store(&ws.counters[g], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_s_waitcnt(0);
+    t2 = __hip_atomic_fetch_add(&ws.counters[kMaxGroups], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
-  __syncthreads();
-  if (threadIdx.x == 0) {
+  t2 = __shfl(t2, 0, 64);
+  if (t2 != ng - 1) return false;
 #pragma unroll
-    for (int v = 0; v < NV; ++v) {
-      double t = lds[v * kWaves];
-      for (int w = 1; w < kWaves; ++w) t += lds[v * kWaves + w];
-      res[v] = t;
-    }
-    *counter = 0u;
+  for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+  for (unsigned i = lane; i < ng; i += 64) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) acc[v] += fetch(&ws.gpartials[v * ng + i]);
   }
-  return true;
+#pragma unroll
+  for (int m = 1; m < 64; m <<= 1) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) acc[v] += shfl_xor_d(acc[v], m);
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) res[v] = acc[v];
+    __hip_atomic_store(&ws.counters[kMaxGroups], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  return lane == 0;
 }
 
 }  // namespace nt2h

@@ -1,11 +1,13 @@
 // Whole-table folds (NT2 inner_fold_ with plus / Zero: sum, globalsum, dot)
 // as one single-pass HIP kernel each.
 //
-// Each lane accumulates a grid-strided slice with 128-bit loads, the four
-// waves of a workgroup combine through DPP shuffles and LDS, each block
-// writes one partial, and the last block to finish (atomic ticket) combines
-// all partials in FP64 in block order and resets the ticket — no second
-// launch, no float atomics, the same bits on every run for a given size.
+// Each lane accumulates a grid-strided slice with 128-bit loads (one vector
+// per lane per step, up to 64K workgroups so that the hardware keeps
+// launching waves while others wait on memory), the four waves of a
+// workgroup combine through shuffles and LDS, and a two-level ticket
+// (groups of 64 workgroups, then the groups) combines the partials in FP64
+// in index order — no second launch, no float atomics, the same bits on
+// every run for a given size.
 // NT2's counterpart: scalar fold_step (core/reduction/include/nt2/core/
 // functions/details/fold_step.hpp:31-43), SIMD inner_fold (simd/
 // inner_fold.hpp:44-67) and the OpenMP block fold merged under omp
@@ -75,8 +77,7 @@ template <> struct LaneAcc<NT2H_SUM_PAIRWISE> {
 
 template <int MODE, bool DOT, bool VEC>
 __global__ __launch_bounds__(kBlock) void fold_f32_kernel(const float* __restrict__ x, const float* __restrict__ y,
-                                                          int64_t n, double* partials, unsigned* counter,
-                                                          double* out) {
+                                                          int64_t n, Workspace ws, double* out) {
   LaneAcc<MODE> acc;
   const int64_t stride = (int64_t)gridDim.x * kBlock;
   const int64_t gid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -115,15 +116,14 @@ __global__ __launch_bounds__(kBlock) void fold_f32_kernel(const float* __restric
   Comp c[1] = {acc.finish()};
   double bsum[1];
   block_reduce_comp<1>(c, lds, bsum);
-  if (threadIdx.x == 0) partials[blockIdx.x] = bsum[0];
   double res[1];
-  if (grid_combine<1>(partials, counter, res) && threadIdx.x == 0) out[0] = res[0];
+  if (grid_combine<1>(bsum, ws, res)) out[0] = res[0];
 }
 
 template <bool DOT>
 __global__ __launch_bounds__(kBlock) void fold_f64_kernel(const double* __restrict__ x,
                                                           const double* __restrict__ y, int64_t n,
-                                                          double* partials, unsigned* counter, double* out) {
+                                                          Workspace ws, double* out) {
   double s = 0.0;
   const int64_t stride = (int64_t)gridDim.x * kBlock;
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
@@ -136,33 +136,23 @@ __global__ __launch_bounds__(kBlock) void fold_f64_kernel(const double* __restri
   __shared__ double lds[kWaves];
   if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = s;
   __syncthreads();
+  double bsum[1] = {0.0};
   if (threadIdx.x == 0) {
     double t = lds[0];
     for (int w = 1; w < kWaves; ++w) t += lds[w];
-    partials[blockIdx.x] = t;
+    bsum[0] = t;
   }
   double res[1];
-  if (grid_combine<1>(partials, counter, res) && threadIdx.x == 0) out[0] = res[0];
-}
-
-static unsigned fold_grid(int64_t n, int vec) {
-  // ~2 grid-stride steps per lane at least; at most kMaxReduceBlocks blocks
-  int64_t units = (n + vec - 1) / vec;
-  int64_t b = (units + 2 * kBlock - 1) / (2 * kBlock);
-  if (b < 1) b = 1;
-  if (b > kMaxReduceBlocks) b = kMaxReduceBlocks;
-  return (unsigned)b;
+  if (grid_combine<1>(bsum, ws, res)) out[0] = res[0];
 }
 
 template <int MODE, bool DOT>
 static void launch_f32(bool vec, unsigned g, const float* x, const float* y, int64_t n, const Workspace& ws,
                        double* dst, hipStream_t st) {
   if (vec)
-    hipLaunchKernelGGL((fold_f32_kernel<MODE, DOT, true>), dim3(g), dim3(kBlock), 0, st, x, y, n, ws.partials,
-                       ws.counter, dst);
+    hipLaunchKernelGGL((fold_f32_kernel<MODE, DOT, true>), dim3(g), dim3(kBlock), 0, st, x, y, n, ws, dst);
   else
-    hipLaunchKernelGGL((fold_f32_kernel<MODE, DOT, false>), dim3(g), dim3(kBlock), 0, st, x, y, n, ws.partials,
-                       ws.counter, dst);
+    hipLaunchKernelGGL((fold_f32_kernel<MODE, DOT, false>), dim3(g), dim3(kBlock), 0, st, x, y, n, ws, dst);
 }
 
 static int fold_f32(const float* x, const float* y, int64_t n, int mode, double* out, nt2h_stream_t s) {
@@ -177,7 +167,7 @@ static int fold_f32(const float* x, const float* y, int64_t n, int mode, double*
   NT2H_TRY(get_workspace(s, &ws));
   double* dst = host ? ws.result : out;
   bool vec = y ? aligned16(x, y) : aligned16(x);
-  unsigned g = fold_grid(n, vec ? 4 : 1);
+  unsigned g = fold_blocks(vec ? (n + 3) / 4 : n);
   hipStream_t st = as_hip(s);
   const bool dot = (y != nullptr);
   switch (mode) {
@@ -207,13 +197,11 @@ static int fold_f64(const double* x, const double* y, int64_t n, double* out, nt
   Workspace ws;
   NT2H_TRY(get_workspace(s, &ws));
   double* dst = host ? ws.result : out;
-  unsigned g = fold_grid(n, 1);
+  unsigned g = fold_blocks(n);
   if (y)
-    hipLaunchKernelGGL((fold_f64_kernel<true>), dim3(g), dim3(kBlock), 0, as_hip(s), x, y, n, ws.partials,
-                       ws.counter, dst);
+    hipLaunchKernelGGL((fold_f64_kernel<true>), dim3(g), dim3(kBlock), 0, as_hip(s), x, y, n, ws, dst);
   else
-    hipLaunchKernelGGL((fold_f64_kernel<false>), dim3(g), dim3(kBlock), 0, as_hip(s), x, y, n, ws.partials,
-                       ws.counter, dst);
+    hipLaunchKernelGGL((fold_f64_kernel<false>), dim3(g), dim3(kBlock), 0, as_hip(s), x, y, n, ws, dst);
   NT2H_HIP(hipGetLastError());
   return deliver(s, out, ws.result, 1, host);
 }

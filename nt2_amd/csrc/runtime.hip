@@ -34,12 +34,12 @@ int get_workspace(nt2h_stream_t s, Workspace* ws) {
     return NT2H_OK;
   }
   Workspace w{};
-  size_t np = (size_t)kMaxReduceBlocks * kMaxPartials * sizeof(double);
-  NT2H_HIP(hipMalloc((void**)&w.partials, np));
-  NT2H_HIP(hipMalloc((void**)&w.counter, 256));
+  NT2H_HIP(hipMalloc((void**)&w.partials, (size_t)kMaxPartials * kMaxFoldBlocks * sizeof(double)));
+  NT2H_HIP(hipMalloc((void**)&w.gpartials, (size_t)kMaxPartials * kMaxGroups * sizeof(double)));
+  NT2H_HIP(hipMalloc((void**)&w.counters, (size_t)(kMaxGroups + 1) * sizeof(unsigned)));
   NT2H_HIP(hipMalloc((void**)&w.result, kMaxPartials * sizeof(double)));
-  // zero the counter once, synchronously; kernels reset it after each use
-  NT2H_HIP(hipMemset(w.counter, 0, 256));
+  // zero the tickets once, synchronously; kernels reset them after each use
+  NT2H_HIP(hipMemset(w.counters, 0, (size_t)(kMaxGroups + 1) * sizeof(unsigned)));
   g_ws[key] = w;
   *ws = w;
   return NT2H_OK;
