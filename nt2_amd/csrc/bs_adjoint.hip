@@ -35,13 +35,19 @@ struct BSAdjOut {
   float price, dS, dK, dT, dr, dsig;
 };
 
-// One option, float, Fused arithmetic.
+// One option, float, Fused arithmetic.  Beyond the functors' own fusion:
+// sqrt(T) is v_sqrt_f32 (1 ulp), 1/(sig sqrt T) is formed once (refined)
+// and serves d1 and dT, and exp(-d2^2/2) is exp(-d1^2/2) * (S/K) * e^{rT}
+// (the identity behind the adjoint's simplification), so the two erfc
+// evaluations share one exp.
 __device__ __forceinline__ BSAdjOut bs_fwd_adj_one(float S, float K, float Tm, float r, float sig, float w) {
   using P = Fused;
-  float sqT = P::sqrt(Tm);
+  float sqT = __builtin_amdgcn_sqrtf(Tm);
   float sst = sig * sqT;
-  float lsk = log_f<P>(P::div(S, K));
-  float d1 = P::div(P::madd(P::madd(sig, sig * 0.5f, r), Tm, lsk), sst);
+  float rsst = P::rcp(sst);
+  float sk = P::div(S, K);
+  float lsk = log_f<P>(sk);
+  float d1 = P::madd(P::madd(sig, sig * 0.5f, r), Tm, lsk) * rsst;
   float d2 = d1 - sst;
   float ert = exp_f<P>(-r * Tm);
   float kert = K * ert;
@@ -51,7 +57,8 @@ __device__ __forceinline__ BSAdjOut bs_fwd_adj_one(float S, float K, float Tm, f
   float r1 = erfc_pos<P>(__builtin_fabsf(t1), &e1);
   float n1 = 0.5f * ((t1 < 0.0f) ? 2.0f - r1 : r1);
   float t2 = -kInvSqrt2 * d2;
-  float r2 = erfc_pos<P>(__builtin_fabsf(t2));
+  float e2 = e1 * (sk * P::rcp(ert));
+  float r2 = erfc_pos_e<P>(__builtin_fabsf(t2), e2);
   float n2 = 0.5f * ((t2 < 0.0f) ? 2.0f - r2 : r2);
   float phi1 = e1 * kInvSqrt2Pi;
   float kn2 = kert * n2;
@@ -60,7 +67,7 @@ __device__ __forceinline__ BSAdjOut bs_fwd_adj_one(float S, float K, float Tm, f
   o.price = P::madd(S, n1, -kn2);
   o.dS = w * n1;
   o.dK = -(w * (ert * n2));
-  o.dT = w * P::madd(sphi * sig, P::div(0.5f, sqT), r * kn2);
+  o.dT = w * P::madd(sphi * (0.5f * sig * sig), rsst, r * kn2);  // S phi sig / (2 sqrt T) = S phi sig^2 / (2 sig sqrt T)
   o.dr = w * (Tm * kn2);
   o.dsig = w * (sphi * sqT);
   return o;
@@ -105,6 +112,27 @@ struct BSAdjArgsD {
   double* out[6];
 };
 
+// Four options (one 16-byte vector per stream) of the float kernel.
+__device__ __forceinline__ void bs_vec4(const BSAdjArgs& a, int64_t i, f32x4 (&o)[6]) {
+  f32x4 v[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) v[k] = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(a.in[k]) + i);
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    BSAdjOut r = bs_fwd_adj_one(v[0][e], v[1][e], v[2][e], v[3][e], v[4][e], v[5][e]);
+    o[0][e] = r.price;
+    o[1][e] = r.dS;
+    o[2][e] = r.dK;
+    o[3][e] = r.dT;
+    o[4][e] = r.dr;
+    o[5][e] = r.dsig;
+  }
+}
+__device__ __forceinline__ void bs_store4(const BSAdjArgs& a, int64_t i, const f32x4 (&o)[6]) {
+#pragma unroll
+  for (int k = 0; k < 6; ++k) __builtin_nontemporal_store(o[k], reinterpret_cast<f32x4*>(a.out[k]) + i);
+}
+
 template <bool VEC, bool TOTALS>
 __global__ __launch_bounds__(kBlock) void bs_fwd_adj_f32_kernel(BSAdjArgs a, int64_t n, Workspace ws,
                                                                 double* totals) {
@@ -115,27 +143,13 @@ __global__ __launch_bounds__(kBlock) void bs_fwd_adj_f32_kernel(BSAdjArgs a, int
   if (VEC) {
     const int64_t nv = n / 4;
     for (int64_t i = gid; i < nv; i += stride) {
-      f32x4 v[6];
-#pragma unroll
-      for (int k = 0; k < 6; ++k) v[k] = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(a.in[k]) + i);
       f32x4 o[6];
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        BSAdjOut r = bs_fwd_adj_one(((const float*)&v[0])[e], ((const float*)&v[1])[e], ((const float*)&v[2])[e],
-                                    ((const float*)&v[3])[e], ((const float*)&v[4])[e], ((const float*)&v[5])[e]);
-        ((float*)&o[0])[e] = r.price;
-        ((float*)&o[1])[e] = r.dS;
-        ((float*)&o[2])[e] = r.dK;
-        ((float*)&o[3])[e] = r.dT;
-        ((float*)&o[4])[e] = r.dr;
-        ((float*)&o[5])[e] = r.dsig;
-        if (TOTALS) {
-          acc[0].add(r.dr);
-          acc[1].add(r.dsig);
-        }
+      bs_vec4(a, i, o);
+      if (TOTALS) {  // pairwise within the vector, then one compensated add
+        acc[0].add((o[4][0] + o[4][1]) + (o[4][2] + o[4][3]));
+        acc[1].add((o[5][0] + o[5][1]) + (o[5][2] + o[5][3]));
       }
-#pragma unroll
-      for (int k = 0; k < 6; ++k) __builtin_nontemporal_store(o[k], reinterpret_cast<f32x4*>(a.out[k]) + i);
+      bs_store4(a, i, o);
     }
     done = nv * 4;
   }

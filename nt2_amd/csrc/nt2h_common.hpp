@@ -19,6 +19,9 @@ constexpr int kMaxFoldBlocks = 65536;  // workgroups of one fold launch (partial
 constexpr int kGroup = 64;            // workgroups combined per first-level ticket
 constexpr int kMaxGroups = kMaxFoldBlocks / kGroup;
 constexpr int kMaxPartials = 4;       // accumulators per block in one fold
+// ticket counters sit 256 bytes apart so that groups never contend for one
+// cache line (atomics to one line serialise at the coherence point)
+constexpr int kCounterStride = 64;
 
 // ---------------------------------------------------------------- host
 void set_hip_error(hipError_t e);
@@ -63,8 +66,8 @@ inline unsigned stream_grid(int64_t n, int vec, int64_t max_blocks = 65536) {
 
 // Reduction workspace of one (device, stream): per-block partials
 // (kMaxPartials x kMaxFoldBlocks doubles), per-group partials (kMaxPartials x
-// kMaxGroups), kMaxGroups + 1 ticket counters that the kernels reset after
-// use, and a result slot for host-pointer outputs.
+// kMaxGroups), kMaxGroups + 1 ticket counters (kCounterStride apart) that the
+// kernels reset after use, and a result slot for host-pointer outputs.
 struct Workspace {
   double* partials;
   double* gpartials;
@@ -167,9 +170,20 @@ __device__ __forceinline__ double fetch(const double* p) {
 // sums the group partials in group order.  Every ticket counter is reset by
 // the block that consumed it.  Returns true in lane 0 of the final block,
 // with the totals in res[0..NV).
-template <int NV>
-__device__ __forceinline__ bool grid_combine(const double (&bsum)[NV], const Workspace& ws, double* res) {
-  if (threadIdx.x >= 64) return false;
+// `between` runs once in every thread: in waves 1.. before they leave, in
+// wave 0 after the ticket atomic is issued and before its result is used —
+// the place for a kernel's deferred output stores, so that the ticket's
+// wait covers only the partials and not the bulk stores.
+struct NoDeferred {
+  __device__ __forceinline__ void operator()() const {}
+};
+template <int NV, class Between = NoDeferred>
+__device__ __forceinline__ bool grid_combine(const double (&bsum)[NV], const Workspace& ws, double* res,
+                                             Between between = Between()) {
+  if (threadIdx.x >= 64) {
+    between();
+    return false;
+  }
   const unsigned nb = gridDim.x, b = blockIdx.x;
   const unsigned ng = (nb + kGroup - 1) / kGroup, g = b / kGroup;
   const unsigned gsize = (nb - g * kGroup) < (unsigned)kGroup ? (nb - g * kGroup) : (unsigned)kGroup;
@@ -179,8 +193,9 @@ __device__ __forceinline__ bool grid_combine(const double (&bsum)[NV], const Wor
 #pragma unroll
     for (int v = 0; v < NV; ++v) publish(&ws.partials[v * nb + b], bsum[v]);
     __builtin_amdgcn_s_waitcnt(0);
-    t = __hip_atomic_fetch_add(&ws.counters[g], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    t = __hip_atomic_fetch_add(&ws.counters[g * kCounterStride], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
+  between();
   t = __shfl(t, 0, 64);
   if (t != gsize - 1) return false;
   double acc[NV];
@@ -195,9 +210,9 @@ __device__ __forceinline__ bool grid_combine(const double (&bsum)[NV], const Wor
   if (lane == 0) {
 #pragma unroll
     for (int v = 0; v < NV; ++v) publish(&ws.gpartials[v * ng + g], acc[v]);
-    __hip_atomic_store(&ws.counters[g], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&ws.counters[g * kCounterStride], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __builtin_amdgcn_s_waitcnt(0);
-    t2 = __hip_atomic_fetch_add(&ws.counters[kMaxGroups], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    t2 = __hip_atomic_fetch_add(&ws.counters[kMaxGroups * kCounterStride], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   t2 = __shfl(t2, 0, 64);
   if (t2 != ng - 1) return false;
@@ -215,7 +230,7 @@ __device__ __forceinline__ bool grid_combine(const double (&bsum)[NV], const Wor
   if (lane == 0) {
 #pragma unroll
     for (int v = 0; v < NV; ++v) res[v] = acc[v];
-    __hip_atomic_store(&ws.counters[kMaxGroups], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&ws.counters[kMaxGroups * kCounterStride], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   return lane == 0;
 }

@@ -222,6 +222,19 @@ template <class P> NT2H_FD float erfc_pos(float x, float* e2 = nullptr) {
   float r = (x < 0x1.555556p-1f) ? small : big;
   return (x == kInfF) ? 0.0f : r;
 }
+// erfc(x) for x >= 0 given ex = exp(-x^2) computed elsewhere (a kernel that
+// knows it from an identity); same branches and polynomials as erfc_pos.
+template <class P> NT2H_FD float erfc_pos_e(float x, float ex) {
+  float z = P::div(x, 1.0f + x);
+  float small = (1.0f - z) * horner<P>(z, 0x1.551caap+4f, 0x1.36ad00p+1f, -0x1.21b2acp+1f, 0x1.59fe76p-1f,
+                                       -0x1.2d930ap+1f, -0x1.f92508p+0f, -0x1.422d30p+0f, -0x1.06dafcp-3f,
+                                       0x1.fffffcp-1f);
+  float big = ex * horner<P>(z - 0.4f, -0x1.506c22p+1f, 0x1.acfc76p+2f, -0x1.9a350ap+2f, 0x1.a160c6p+1f,
+                             -0x1.5d50cap+0f, 0x1.400de0p-3f, 0x1.22eb8ep-3f, 0x1.8994dcp-2f, 0x1.487050p-3f,
+                             -0x1.2314c4p+0f, 0x1.141d16p-1f);
+  float r = (x < 0x1.555556p-1f) ? small : big;
+  return (x == kInfF) ? 0.0f : r;
+}
 template <class P> NT2H_FD float erfc_f(float a0) {
   float r = erfc_pos<P>(__builtin_fabsf(a0));
   r = (a0 < 0.0f) ? 2.0f - r : r;
