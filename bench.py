@@ -62,7 +62,7 @@ def _cpu_baseline(inputs_np, budget_s=12.0):
         nt2ref.cpu_bs_fwd_adj_f32(ins, outs)
         reps += 1
         el = time.perf_counter() - t0
-        if el >= budget_s or reps >= 50:
+        if el >= budget_s or reps >= 2000:
             break
     return {
         "value": n * reps / el,
@@ -74,12 +74,162 @@ def _cpu_baseline(inputs_np, budget_s=12.0):
     }
 
 
+def _events(stream, fn, reps):
+    import torch
+
+    fn()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(stream)
+    for _ in range(reps):
+        fn()
+    b.record(stream)
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / reps / 1e3  # seconds per launch
+
+
+def _ulp_vs_fp64(y32, y64):
+    """Float ulps of y32 against y64 where y64 is a normal float (|y64| >= 2^-126);
+    NT2's functors flush below that (exp builds 2^k from exponent bits), so the
+    subnormal band is reported apart as a count and a maximum absolute error."""
+    import torch
+
+    tiny = torch.finfo(torch.float32).tiny
+    normal = (y64.abs() >= tiny) | (y64 == 0)
+    both_nan = torch.isnan(y32) & torch.isnan(y64)
+    diff = (y32.double() - y64).abs()
+    same = (y32.double() == y64) | both_nan
+    _, e = torch.frexp(torch.clamp(y64.abs(), min=tiny))
+    sp = torch.ldexp(torch.ones_like(y64), (e - 24).to(torch.int32))
+    u = torch.where(same | ~normal, torch.zeros_like(diff), diff / sp)
+    rel = torch.where(same | ~normal, torch.zeros_like(diff), diff / torch.clamp(y64.abs(), min=1e-300))
+    sub = ~normal
+    sub_abs = float(torch.where(sub, diff, torch.zeros_like(diff)).max()) if bool(sub.any()) else 0.0
+    return float(u.max()), int(u.argmax()), float(rel.max()), int(sub.sum()), sub_abs
+
+
+def report_sweep(dev, stream, n=1 << 28):
+    """Precision sweep: each functor over 2^28 floats, FP32 (EXACT, FUSED) vs FP64 on the GPU."""
+    import torch
+
+    from nt2_amd import EXACT, FUSED, SUM_KAHAN, SUM_NAIVE, SUM_PAIRWISE, ops, synth
+
+    lines = []
+    for fn in ("exp", "log", "sqrt", "erfc", "tanh", "sin", "cos"):
+        x = torch.from_numpy(synth.sweep_args(fn, n)).to(dev)
+        x64 = x.double()
+        y64 = ops.map(fn, x64, EXACT)
+        t64 = _events(stream, lambda: ops.map(fn, x64, EXACT, out=y64), 3)
+        rec = {"report": "precision_sweep", "functor": fn, "n": n, "args": synth.SWEEP[fn],
+               "f64_us": t64 * 1e6, "f64_gbps": 16 * n / t64 / 1e9}
+        for name, pr in (("exact", EXACT), ("fused", FUSED)):
+            y = ops.map(fn, x, pr)
+            t = _events(stream, lambda: ops.map(fn, x, pr, out=y), 5)
+            u, am, rel, nsub, sub_abs = _ulp_vs_fp64(y, y64)
+            rec[name] = {"us": t * 1e6, "gbps": 8 * n / t / 1e9, "hbm_frac": 8 * n / t / 8e12,
+                         "max_ulp_vs_fp64": u, "worst_arg": float(x[am]), "max_rel_vs_fp64": rel,
+                         "subnormal_band_count": nsub, "subnormal_band_max_abs_err": sub_abs,
+                         "speedup_vs_f64": t64 / t}
+            del y
+        lines.append(rec)
+        del x, x64, y64
+        torch.cuda.empty_cache()
+    a_np, b_np = synth.sumdot_inputs(n)
+    a, b = torch.from_numpy(a_np).to(dev), torch.from_numpy(b_np).to(dev)
+    a64, b64 = a.double(), b.double()
+    s64 = ops.sum(a64).item()
+    d64 = ops.dot(a64, b64).item()
+    t64s = _events(stream, lambda: ops.sum(a64), 5)
+    t64d = _events(stream, lambda: ops.dot(a64, b64), 5)
+    rec = {"report": "sum_dot", "n": n, "values": "N(1, 1e-3)", "f64": {"sum": s64, "dot": d64,
+           "sum_us": t64s * 1e6, "dot_us": t64d * 1e6}}
+    for name, mode in (("naive", SUM_NAIVE), ("kahan", SUM_KAHAN), ("pairwise", SUM_PAIRWISE)):
+        s = ops.sum(a, mode).item()
+        d = ops.dot(a, b, mode).item()
+        ts = _events(stream, lambda: ops.sum(a, mode), 5)
+        td = _events(stream, lambda: ops.dot(a, b, mode), 5)
+        ulp_s = abs(float(np.float32(s)) - s64) / float(np.spacing(np.float32(s64)))
+        ulp_d = abs(float(np.float32(d)) - d64) / float(np.spacing(np.float32(d64)))
+        rec[name] = {"sum_rel_err": abs(s - s64) / abs(s64), "sum_ulp_f32": ulp_s, "sum_us": ts * 1e6,
+                     "sum_gbps": 4 * n / ts / 1e9, "dot_rel_err": abs(d - d64) / abs(d64), "dot_ulp_f32": ulp_d,
+                     "dot_us": td * 1e6, "dot_gbps": 8 * n / td / 1e9}
+    lines.append(rec)
+    return lines
+
+
+def report_bs_fwd(dev, stream, n=1 << 26):
+    """Black-Scholes forward (call, put) over 2^26 options: FP32 EXACT / FUSED vs FP64."""
+    import torch
+
+    from nt2_amd import EXACT, FUSED, ops, synth
+
+    d = synth.bs_inputs(n)
+    g = [torch.from_numpy(d[k]).to(dev) for k in ("S", "K", "T", "r", "sig")]
+    g64 = [t.double() for t in g]
+    c64, p64 = ops.bs_fwd(*g64, precision=EXACT)
+    t64 = _events(stream, lambda: ops.bs_fwd(*g64, precision=EXACT), 5)
+    rec = {"report": "bs_fwd", "options": n, "f64_us": t64 * 1e6, "f64_gbps": 56 * n / t64 / 1e9}
+    S = g64[0]
+    for name, pr in (("exact", EXACT), ("fused", FUSED)):
+        c, p = ops.bs_fwd(*g, precision=pr)
+        t = _events(stream, lambda: ops.bs_fwd(*g, precision=pr), 10)
+        sc = torch.maximum(c64.abs(), 1e-6 * S)
+        sp_ = torch.maximum(p64.abs(), 1e-6 * S)
+        rec[name] = {"us": t * 1e6, "options_per_s": n / t, "gbps": 28 * n / t / 1e9, "hbm_frac": 28 * n / t / 8e12,
+                     "call_scaled_err": float(((c.double() - c64).abs() / sc).max()),
+                     "put_scaled_err": float(((p.double() - p64).abs() / sp_).max()),
+                     "speedup_vs_f64": t64 / t}
+    return [rec]
+
+
+def report_saxpy(dev, stream, n=1 << 20):
+    import torch
+
+    from nt2_amd import SUM_KAHAN, ops, synth
+
+    x_np, y_np = synth.saxpy_inputs(n)
+    x, y = torch.from_numpy(x_np).to(dev), torch.from_numpy(y_np).to(dev)
+    t = _events(stream, lambda: ops.saxpy_(2.5, x, y), 50)
+    td = _events(stream, lambda: ops.dot(x, y, SUM_KAHAN), 50)
+    return [{"report": "saxpy_sdot", "n": n, "saxpy_us": t * 1e6, "saxpy_gbps": 12 * n / t / 1e9,
+             "sdot_us": td * 1e6, "sdot_gbps": 8 * n / td / 1e9,
+             "note": "2^20 elements: launch-latency bound, not a bandwidth figure"}]
+
+
+def report_colfold(dev, stream, rows=16384, cols=8192):
+    """Column fold + adjoint over 16384 x 8192: time, and error vs a torch FP64 evaluation on 64 columns."""
+    import torch
+
+    from nt2_amd import ops, synth
+
+    X_np, s_np, k_np, p_np = synth.colfold_inputs(rows, cols)
+    X = torch.from_numpy(X_np).to(dev)
+    s, k, p = (torch.from_numpy(a).to(dev) for a in (s_np, k_np, p_np))
+    z, ds, dk, dp = ops.colfold_gc(X, s, k, p)
+    t = _events(stream, lambda: ops.colfold_gc(X, s, k, p), 5)
+    tf = _events(stream, lambda: ops.colfold_gc(X, s, k, p, grad=False), 5)
+    sel = torch.arange(0, cols, cols // 64, device=dev)
+    Xd, sd, kd, pd = X[sel].double(), s[sel].double()[:, None], k[sel].double()[:, None], p[sel].double()[:, None]
+    e = torch.exp(-Xd * Xd / (2 * sd * sd))
+    arg = kd * Xd + pd
+    zc, sn = e * torch.cos(arg), e * torch.sin(arg)
+    ref = [zc.sum(1), (zc * Xd * Xd).sum(1) / sd[:, 0] ** 3, -(sn * Xd).sum(1), -sn.sum(1)]
+    mag = [e.sum(1), (e * Xd * Xd).sum(1) / sd[:, 0] ** 3, (e * Xd.abs()).sum(1), e.sum(1)]
+    errs = [float(((g[sel].double() - r).abs() / m).max()) for g, r, m in zip((z, ds, dk, dp), ref, mag)]
+    return [{"report": "colfold", "rows": rows, "cols": cols, "fwd_adj_us": t * 1e6, "fwd_only_us": tf * 1e6,
+             "fwd_adj_gbps": 4 * rows * cols / t / 1e9, "hbm_frac": 4 * rows * cols / t / 8e12,
+             "elements_per_s": rows * cols / t,
+             "err_vs_fp64_over_sum_abs_terms": dict(zip(("z", "dz_ds", "dz_dk", "dz_dp"), errs))}]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--report", default="", help="extra report lines after the headline: "
+                    "comma list of sweep,bs_fwd,saxpy,colfold (or all)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -109,17 +259,18 @@ def main():
         step()
     torch.cuda.synchronize()
 
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    # K back-to-back launches bracketed by synchronisation; HIP events on the
+    # launch stream give the kernel time per launch (gaps included)
+    e_start, e_end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    e_start.record(stream)
     for i in range(args.steps):
-        ev[i][0].record(stream)
         res = step()
-        ev[i][1].record(stream)
+    e_end.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    kern_ms = [a.elapsed_time(b) for a, b in ev]
-    avg_kern_s = sum(kern_ms) / len(kern_ms) / 1e3
+    avg_kern_s = e_start.elapsed_time(e_end) / 1e3 / args.steps
     ms_per_step = wall / args.steps * 1e3
     value = n * args.steps / wall
     achieved = BYTES_PER_OPTION * n / avg_kern_s / 1e9
@@ -237,6 +388,17 @@ def main():
         line["cpu_baseline"] = _cpu_baseline(host)
         line["cpu_baseline"]["gpu_over_cpu"] = value / line["cpu_baseline"]["value"]
     print(json.dumps(line), flush=True)
+    want = set(args.report.split(",")) if args.report else set()
+    if "all" in want:
+        want = {"saxpy", "bs_fwd", "sweep", "colfold"}
+    del g, outs, g64, outs64
+    torch.cuda.empty_cache()
+    for name, fn in (("saxpy", report_saxpy), ("bs_fwd", report_bs_fwd), ("colfold", report_colfold),
+                     ("sweep", report_sweep)):
+        if name in want:
+            for rec in fn(dev, stream):
+                print(json.dumps(rec), flush=True)
+            torch.cuda.empty_cache()
 
 
 if __name__ == "__main__":

@@ -51,10 +51,11 @@ __device__ __forceinline__ BSAdjOut bs_fwd_adj_one(float S, float K, float Tm, f
   float d2 = d1 - sst;
   float ert = exp_f<P>(-r * Tm);
   float kert = K * ert;
-  // N(d1) and exp(-d1^2/2) from one erfc core
+  // N(d1) and phi(d1) share exp(-d1^2/2), formed from d1 with d1^2 split
+  // exactly (not from t1 = -d1/sqrt2, whose rounding would cost ~2 t1^2 ulp)
   float t1 = -kInvSqrt2 * d1;
-  float e1;
-  float r1 = erfc_pos<P>(__builtin_fabsf(t1), &e1);
+  float e1 = exp_neg_half_square<P>(d1);
+  float r1 = erfc_pos_e<P>(__builtin_fabsf(t1), e1);
   float n1 = 0.5f * ((t1 < 0.0f) ? 2.0f - r1 : r1);
   float t2 = -kInvSqrt2 * d2;
   float e2 = e1 * (sk * P::rcp(ert));

@@ -178,7 +178,7 @@ static bool op_valid(int op, bool is_float) {
 // separately, with identical results).
 template <class P> __device__ __forceinline__ void ncdf_pair(float d, float& nd, float& nmd) {
   float t = -0x1.6a09e6p-1f * d;
-  float r = erfc_pos<P>(__builtin_fabsf(t));
+  float r = P::fused ? erfc_pos_e<P>(__builtin_fabsf(t), exp_neg_half_square<P>(d)) : erfc_pos<P>(__builtin_fabsf(t));
   float rr = 2.0f - r;
   nd = 0.5f * ((t < 0.0f) ? rr : r);
   nmd = 0.5f * ((t > 0.0f) ? rr : r);

@@ -203,6 +203,34 @@ template <class P> NT2H_FD double log_d(double a0) {
 }
 
 // ============================================================ erf / erfc
+// exp(-x^2).  Exact policy: NT2's exp(-sqr(x)), whose rounding of x^2 costs
+// up to ~2|x| ulp(x^2) relative (about 70 ulp near x = 8).  Fused policy:
+// x^2 = h + l exactly (FMA) and exp(-h-l) = exp(-h) (1 - l) to first order,
+// which removes that error.
+template <class P> NT2H_FD float exp_neg_of_square(float x) {
+  float h = x * x;
+  float e = exp_neg_f<P>(-h);
+  if (P::fused) {  // (e == 0 covers x^2 = inf, where l would be NaN)
+    float l = __builtin_fmaf(x, x, -h);
+    e = (e != 0.0f) ? __builtin_fmaf(-l, e, e) : e;
+  }
+  return e;
+}
+
+// exp(-d^2/2), Fused policy with d^2 split exactly (see exp_neg_of_square).
+// normcdf(d) = erfc(|t|)/2 with t = -d/sqrt2: forming t in float costs up to
+// 2 t^2 ulp of relative error in exp(-t^2) (~160 ulp at d = -12.7), while
+// exp(-d^2/2) from d itself does not.
+template <class P> NT2H_FD float exp_neg_half_square(float d) {
+  float h = d * d;
+  float e = exp_neg_f<P>(-0.5f * h);
+  if (P::fused) {
+    float l = __builtin_fmaf(d, d, -h);
+    e = (e != 0.0f) ? __builtin_fmaf(-0.5f * l, e, e) : e;
+  }
+  return e;
+}
+
 // NT2 float erfc (scalar/erfc.hpp:77-105, details/erf_kernel.hpp:29-89):
 // z = x/(1+x); x < 2/3: (1-z)*P8(z); else exp(-x^2)*P10(z-0.4).  Both
 // branches are evaluated and selected (waves mix them anyway).
@@ -214,7 +242,7 @@ template <class P> NT2H_FD float erfc_pos(float x, float* e2 = nullptr) {
                                        -0x1.2d930ap+1f, -0x1.f92508p+0f, -0x1.422d30p+0f, -0x1.06dafcp-3f,
                                        0x1.fffffcp-1f);
   float zz = z - 0.4f;
-  float ex = exp_neg_f<P>(-(x * x));
+  float ex = exp_neg_of_square<P>(x);
   if (e2) *e2 = ex;
   float big = ex * horner<P>(zz, -0x1.506c22p+1f, 0x1.acfc76p+2f, -0x1.9a350ap+2f, 0x1.a160c6p+1f,
                              -0x1.5d50cap+0f, 0x1.400de0p-3f, 0x1.22eb8ep-3f, 0x1.8994dcp-2f, 0x1.487050p-3f,
@@ -246,7 +274,7 @@ template <class P> NT2H_FD float erf_f(float a0) {
   float small = a0 * horner<P>(x * x, -0x1.3f9076p-11f, 0x1.488d1ap-8f, -0x1.b6c3e8p-6f, 0x1.ce1e44p-4f,
                                -0x1.8126fcp-2f, 0x1.20dd74p+0f);
   float z = P::div(x, 1.0f + x) - 0.4f;
-  float r2 = 1.0f - exp_neg_f<P>(-(x * x)) * horner<P>(z, -0x1.506c22p+1f, 0x1.acfc76p+2f, -0x1.9a350ap+2f,
+  float r2 = 1.0f - exp_neg_of_square<P>(x) * horner<P>(z, -0x1.506c22p+1f, 0x1.acfc76p+2f, -0x1.9a350ap+2f,
                                                        0x1.a160c6p+1f, -0x1.5d50cap+0f, 0x1.400de0p-3f,
                                                        0x1.22eb8ep-3f, 0x1.8994dcp-2f, 0x1.487050p-3f,
                                                        -0x1.2314c4p+0f, 0x1.141d16p-1f);
@@ -319,10 +347,27 @@ template <class P> NT2H_FD double erf_d(double a0) {
 
 // ================================================ normcdf & friends
 // generic/normcdf.hpp:34: Half * erfc(-Sqrt_2o_2 * a0)
-template <class P> NT2H_FD float normcdf_f(float a0) { return 0.5f * erfc_f<P>(-0x1.6a09e6p-1f * a0); }
+template <class P> NT2H_FD float normcdf_f(float a0) {
+  if (P::fused) {
+    float t = -0x1.6a09e6p-1f * a0;
+    float r = erfc_pos_e<P>(__builtin_fabsf(t), exp_neg_half_square<P>(a0));
+    r = (t < 0.0f) ? 2.0f - r : r;
+    return (a0 != a0) ? a0 : 0.5f * r;
+  }
+  return 0.5f * erfc_f<P>(-0x1.6a09e6p-1f * a0);
+}
 template <class P> NT2H_FD double normcdf_d(double a0) { return 0.5 * erfc_d<P>(-0x1.6a09e667f3bcdp-1 * a0); }
 // generic/normpdf.hpp:31: exp(Mhalf*sqr(a0)) * Invsqrt_2pi
-template <class P> NT2H_FD float normpdf_f(float a0) { return exp_f<P>(-0.5f * (a0 * a0)) * 0x1.988454p-2f; }
+template <class P> NT2H_FD float normpdf_f(float a0) {
+  if (P::fused) {  // exp(-x^2/2) with x^2 split as in exp_neg_of_square
+    float h = a0 * a0;
+    float l = __builtin_fmaf(a0, a0, -h);
+    float e = exp_f<P>(-0.5f * h);
+    e = (e != 0.0f) ? __builtin_fmaf(-0.5f * l, e, e) : e;
+    return e * 0x1.988454p-2f;
+  }
+  return exp_f<P>(-0.5f * (a0 * a0)) * 0x1.988454p-2f;
+}
 template <class P> NT2H_FD double normpdf_d(double a0) {
   return exp_d<P>(-0.5 * (a0 * a0)) * 0x1.9884533d43651p-2;
 }

@@ -28,9 +28,17 @@ SPECIALS = [0.0, -0.0, 1.0, -1.0, 0.5, np.inf, -np.inf, np.nan, 1e-45, -1e-45, 1
             62.83185, 201.06, 823549.6, 823549.7, 1e6, 3e7, 3.4e38, -3.4e38, 9.3, 10.05, 127.0, -127.0, 38.2, -37.9]
 
 # FUSED arithmetic: max std-ULP distance to the NT2 scalar result, per functor
-# (erfc's own error is up to 16 ulp on [2/3, inf): FMA moves it by as much)
-FUSED_ULP = {"exp": 2, "exp2": 2, "exp10": 2, "log": 2, "sqrt": 0, "erf": 4, "erfc": 16, "normcdf": 16,
-             "normpdf": 4, "fastnormcdf": 16, "tanh": 4, "sin": 4, "cos": 4}
+# (erfc's own error is up to 16 ulp on [2/3, inf): FMA moves it by as much;
+# in the tails NT2's float exp(-sqr(x)) is ~70 ulp off and the FUSED split of
+# x^2 is not, so the distance to NT2 there is NT2's own error)
+# (normcdf: NT2 rounds t = -d/sqrt2 before exp(-t^2), ~190 ulp off at d = -12.7;
+# the FUSED normcdf forms exp(-d^2/2) from d and is not)
+FUSED_ULP = {"exp": 2, "exp2": 2, "exp10": 2, "log": 2, "sqrt": 0, "erf": 4, "erfc": 80, "normcdf": 260,
+             "normpdf": 80, "fastnormcdf": 16, "tanh": 4, "sin": 4, "cos": 4}
+# FUSED arithmetic against the FP64 evaluation (NT2's double functors), in
+# float ulps at the FP64 value (floored at the smallest normal)
+FUSED_TRUTH_ULP = {"exp": 1.5, "exp2": 1.5, "exp10": 2.5, "log": 1.5, "sqrt": 0.5, "erf": 3, "erfc": 18,
+                   "normcdf": 18, "normpdf": 3, "tanh": 2.5, "sin": 2, "cos": 2}
 
 
 def _args(fn, n, dtype=np.float32):
@@ -120,6 +128,26 @@ def test_map_f32_fused_bound(dev, ref, fn):
         m = (np.abs(want) >= np.float32(1.1754944e-38)) & (np.abs(got) >= np.float32(1.1754944e-38))
         st = ref.cmp_f32(got[m], want[m])
     assert st["max_std_ulp"] <= FUSED_ULP[fn], st
+    if fn in FUSED_TRUTH_ULP:
+        x64 = x.astype(np.float64)
+        import scipy.special as sp
+
+        # FP64 truth from numpy/scipy (NT2's double erf returns +1 below -6,
+        # scalar/erf.hpp:69, so it cannot serve as truth there)
+        with np.errstate(all="ignore"):
+            truth = {"exp": np.exp, "exp2": np.exp2, "exp10": lambda v: np.power(10.0, v), "log": np.log,
+                     "sqrt": np.sqrt, "erf": sp.erf, "erfc": sp.erfc, "normcdf": sp.ndtr,
+                     "normpdf": lambda v: np.exp(-0.5 * v * v) / np.sqrt(2 * np.pi), "tanh": np.tanh,
+                     "sin": np.sin, "cos": np.cos}[fn](x64)
+        fin = np.isfinite(truth) & np.isfinite(got)
+        # NT2 flushes exp below 2^-126 and clamps at its Minlog/Maxlog (the
+        # FUSED functors keep those semantics): skip the arguments whose true
+        # result leaves the normal float range or that NT2 sends to 0 / inf
+        fin &= (np.abs(truth) >= 1.1754944e-38) | (truth == 0)
+        fin &= np.abs(truth) <= 3.4e38
+        fin &= ~(((want == 0) | np.isinf(want)) & (truth != 0))
+        st2 = ref.ulp_stats_f32(got[fin], truth[fin])
+        assert st2["max_ulp"] <= FUSED_TRUTH_ULP[fn], (st2, x[fin][st2["argmax"]])
 
 
 @pytest.mark.parametrize("fn", F64_OPS)
