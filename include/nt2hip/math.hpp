@@ -1,4 +1,4 @@
-// Device functors for the fused table-expression kernels (gfx950).
+// Device functors of the NT2 table-expression evaluator (gfx950).
 //
 // Each functor follows the algorithm and coefficient table of NT2's scalar
 // functor (cited per function) so that GPU results stay within a few ULP of
@@ -6,9 +6,10 @@
 //
 //   Exact  — every operation rounded separately, IEEE division and square
 //            root: NT2's scalar semantics (its fma/fnms/horner are a*b+c with
-//            two roundings on x86).  This TU family is compiled with
-//            -ffp-contract=off, so a*b+c here really rounds twice and the
-//            result is bit-identical to NT2's scalar path.
+//            two roundings on x86).  Contraction is switched off from this
+//            header on (pragma below, whatever the includer's -ffp-contract),
+//            so a*b+c here really rounds twice and the result is
+//            bit-identical to NT2's scalar path.
 //   Fused  — v_fma_f32/f64 for every multiply-add and a refined hardware
 //            reciprocal for float division: fewer VALU slots, same or better
 //            accuracy, within a stated ULP bound of the Exact result.
@@ -21,6 +22,10 @@
 #include <stdint.h>
 
 #define NT2H_FD __device__ __forceinline__
+
+// No contraction from here to the end of the including translation unit
+// (clang has no push/pop for this on the AMDGPU target).
+#pragma clang fp contract(off)
 
 namespace nt2h {
 

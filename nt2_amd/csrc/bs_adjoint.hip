@@ -24,7 +24,7 @@
 #include <hip/hip_runtime.h>
 
 #include "nt2h_common.hpp"
-#include "nt2h_math.hpp"
+#include "../../include/nt2hip/math.hpp"
 
 namespace nt2h {
 
@@ -169,9 +169,9 @@ __global__ __launch_bounds__(kBlock) void bs_fwd_adj_f32_kernel(BSAdjArgs a, int
   }
   if (TOTALS) {
     __shared__ float lds[2 * 2 * kWaves];
-    double bsum[2];
+    double bsum[2] = {0.0, 0.0};
     block_reduce_comp<2>(acc, lds, bsum);
-    double res[2];
+    double res[2] = {0.0, 0.0};
     if (grid_combine<2>(bsum, ws, res)) {
       totals[0] = res[0];
       totals[1] = res[1];
@@ -218,7 +218,7 @@ __global__ __launch_bounds__(kBlock) void bs_fwd_adj_f64_kernel(BSAdjArgsD a, in
       bsum[0] = t0;
       bsum[1] = t1;
     }
-    double res[2];
+    double res[2] = {0.0, 0.0};
     if (grid_combine<2>(bsum, ws, res)) {
       totals[0] = res[0];
       totals[1] = res[1];

@@ -19,7 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include "nt2h_common.hpp"
-#include "nt2h_math.hpp"
+#include "../../include/nt2hip/math.hpp"
 
 namespace nt2h {
 
@@ -96,7 +96,7 @@ __global__ __launch_bounds__(kBlock) void colfold_kernel(const float* __restrict
 
   constexpr int NV = GRAD ? 4 : 1;
   __shared__ float lds[NV * 2 * kWaves];
-  double tot[NV];
+  double tot[NV] = {};
   block_reduce_comp<NV>(acc.a, lds, tot);
   if (threadIdx.x == 0) {
     z[j] = (float)tot[0];

@@ -11,7 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include "nt2h_common.hpp"
-#include "nt2h_math.hpp"
+#include "../../include/nt2hip/math.hpp"
 
 namespace nt2h {
 

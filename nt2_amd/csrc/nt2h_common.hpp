@@ -6,22 +6,9 @@
 #include <stdint.h>
 
 #include "../../include/nt2hip.h"
+#include "../../include/nt2hip/fold.hpp"
 
 namespace nt2h {
-
-// Native clang vectors: 128-bit loads/stores that the nontemporal builtins accept.
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-typedef double f64x2 __attribute__((ext_vector_type(2)));
-
-constexpr int kBlock = 256;          // 4 wave64 per workgroup
-constexpr int kWaves = kBlock / 64;
-constexpr int kMaxFoldBlocks = 65536;  // workgroups of one fold launch (partials slots)
-constexpr int kGroup = 64;            // workgroups combined per first-level ticket
-constexpr int kMaxGroups = kMaxFoldBlocks / kGroup;
-constexpr int kMaxPartials = 4;       // accumulators per block in one fold
-// ticket counters sit 256 bytes apart so that groups never contend for one
-// cache line (atomics to one line serialise at the coherence point)
-constexpr int kCounterStride = 64;
 
 // ---------------------------------------------------------------- host
 void set_hip_error(hipError_t e);
@@ -64,175 +51,10 @@ inline unsigned stream_grid(int64_t n, int vec, int64_t max_blocks = 65536) {
   return (unsigned)b;
 }
 
-// Reduction workspace of one (device, stream): per-block partials
-// (kMaxPartials x kMaxFoldBlocks doubles), per-group partials (kMaxPartials x
-// kMaxGroups), kMaxGroups + 1 ticket counters (kCounterStride apart) that the
-// kernels reset after use, and a result slot for host-pointer outputs.
-struct Workspace {
-  double* partials;
-  double* gpartials;
-  unsigned* counters;
-  double* result;  // kMaxPartials doubles
-};
-// Workgroups for a fold over `units` vectors, one vector per lane per step.
-inline unsigned fold_blocks(int64_t units) {
-  int64_t b = (units + kBlock - 1) / kBlock;
-  if (b < 1) b = 1;
-  if (b > kMaxFoldBlocks) b = kMaxFoldBlocks;
-  return (unsigned)b;
-}
 int get_workspace(nt2h_stream_t s, Workspace* ws);
 // Deliver nv doubles computed into dev (device memory) to out (device or
 // host memory); for host memory the stream is synchronised.
 int deliver(nt2h_stream_t s, double* out, const double* dev, int nv, bool out_is_host);
 int is_host_pointer(const void* p, bool* host);
-
-// ---------------------------------------------------------------- device
-// Error-free transformation: s + e == a + b exactly (Knuth TwoSum).
-__device__ __forceinline__ void two_sum(float a, float b, float& s, float& e) {
-  s = a + b;
-  float bp = s - a;
-  e = (a - (s - bp)) + (b - bp);
-}
-// Compensated accumulator: value = s + c.
-struct Comp {
-  float s, c;
-  __device__ __forceinline__ void add(float x) {
-    float t, e;
-    two_sum(s, x, t, e);
-    s = t;
-    c += e;
-  }
-  __device__ __forceinline__ void merge(float s2, float c2) {
-    float t, e;
-    two_sum(s, s2, t, e);
-    s = t;
-    c = c + c2 + e;
-  }
-};
-
-__device__ __forceinline__ float shfl_xor_f(float v, int m) { return __shfl_xor(v, m, 64); }
-__device__ __forceinline__ double shfl_xor_d(double v, int m) { return __shfl_xor(v, m, 64); }
-
-// Block-wide fixed-order reduction of NV compensated accumulators; the
-// result (s + c in double) is valid in thread 0.  Uses lds[NV*2*kWaves].
-template <int NV>
-__device__ __forceinline__ void block_reduce_comp(Comp (&acc)[NV], float* lds, double (&out)[NV]) {
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) {
-#pragma unroll
-    for (int v = 0; v < NV; ++v) {
-      float s2 = shfl_xor_f(acc[v].s, m), c2 = shfl_xor_f(acc[v].c, m);
-      acc[v].merge(s2, c2);
-    }
-  }
-  int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  if (lane == 0) {
-#pragma unroll
-    for (int v = 0; v < NV; ++v) {
-      lds[(v * kWaves + wave) * 2 + 0] = acc[v].s;
-      lds[(v * kWaves + wave) * 2 + 1] = acc[v].c;
-    }
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int v = 0; v < NV; ++v) {
-      Comp a{lds[(v * kWaves) * 2], lds[(v * kWaves) * 2 + 1]};
-      for (int w = 1; w < kWaves; ++w) a.merge(lds[(v * kWaves + w) * 2], lds[(v * kWaves + w) * 2 + 1]);
-      out[v] = (double)a.s + (double)a.c;
-    }
-  }
-}
-
-// Cross-block handoff without an agent-scope fence.  On gfx950 a release
-// fence at agent scope writes back the whole L2 of the XCD, which costs
-// more than the fold itself when every block issues it.  Instead each
-// block publishes its partials with agent-scope atomic stores (written
-// through to the point of coherence of all XCDs), waits for them to
-// complete, and takes a ticket with an agent-scope atomic add; the block
-// holding the last ticket reads the partials back with agent-scope atomic
-// loads (which bypass any stale L2 copy).
-__device__ __forceinline__ void publish(double* p, double v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ double fetch(const double* p) {
-  return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Two-level deterministic combine of per-block partials, after
-// block_reduce_comp (or an equivalent) left the block's NV sums in thread 0.
-// Only wave 0 takes part; the other waves leave at once.  Thread 0
-// publishes the block's partials and takes a ticket on its group of kGroup
-// blocks; the block holding the group's last ticket sums the group's
-// partials in block order (one lane per block, fixed shuffle tree), publishes
-// the group partial and takes a ticket on the top counter; the last group
-// sums the group partials in group order.  Every ticket counter is reset by
-// the block that consumed it.  Returns true in lane 0 of the final block,
-// with the totals in res[0..NV).
-// `between` runs once in every thread: in waves 1.. before they leave, in
-// wave 0 after the ticket atomic is issued and before its result is used —
-// the place for a kernel's deferred output stores, so that the ticket's
-// wait covers only the partials and not the bulk stores.
-struct NoDeferred {
-  __device__ __forceinline__ void operator()() const {}
-};
-template <int NV, class Between = NoDeferred>
-__device__ __forceinline__ bool grid_combine(const double (&bsum)[NV], const Workspace& ws, double* res,
-                                             Between between = Between()) {
-  if (threadIdx.x >= 64) {
-    between();
-    return false;
-  }
-  const unsigned nb = gridDim.x, b = blockIdx.x;
-  const unsigned ng = (nb + kGroup - 1) / kGroup, g = b / kGroup;
-  const unsigned gsize = (nb - g * kGroup) < (unsigned)kGroup ? (nb - g * kGroup) : (unsigned)kGroup;
-  const int lane = threadIdx.x;
-  unsigned t = 0;
-  if (lane == 0) {
-#pragma unroll
-    for (int v = 0; v < NV; ++v) publish(&ws.partials[v * nb + b], bsum[v]);
-    __builtin_amdgcn_s_waitcnt(0);
-    t = __hip_atomic_fetch_add(&ws.counters[g * kCounterStride], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-  between();
-  t = __shfl(t, 0, 64);
-  if (t != gsize - 1) return false;
-  double acc[NV];
-#pragma unroll
-  for (int v = 0; v < NV; ++v) acc[v] = ((unsigned)lane < gsize) ? fetch(&ws.partials[v * nb + g * kGroup + lane]) : 0.0;
-#pragma unroll
-  for (int m = 1; m < 64; m <<= 1) {
-#pragma unroll
-    for (int v = 0; v < NV; ++v) acc[v] += shfl_xor_d(acc[v], m);
-  }
-  unsigned t2 = 0;
-  if (lane == 0) {
-#pragma unroll
-    for (int v = 0; v < NV; ++v) publish(&ws.gpartials[v * ng + g], acc[v]);
-    __hip_atomic_store(&ws.counters[g * kCounterStride], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __builtin_amdgcn_s_waitcnt(0);
-    t2 = __hip_atomic_fetch_add(&ws.counters[kMaxGroups * kCounterStride], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-  t2 = __shfl(t2, 0, 64);
-  if (t2 != ng - 1) return false;
-#pragma unroll
-  for (int v = 0; v < NV; ++v) acc[v] = 0.0;
-  for (unsigned i = lane; i < ng; i += 64) {
-#pragma unroll
-    for (int v = 0; v < NV; ++v) acc[v] += fetch(&ws.gpartials[v * ng + i]);
-  }
-#pragma unroll
-  for (int m = 1; m < 64; m <<= 1) {
-#pragma unroll
-    for (int v = 0; v < NV; ++v) acc[v] += shfl_xor_d(acc[v], m);
-  }
-  if (lane == 0) {
-#pragma unroll
-    for (int v = 0; v < NV; ++v) res[v] = acc[v];
-    __hip_atomic_store(&ws.counters[kMaxGroups * kCounterStride], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-  return lane == 0;
-}
 
 }  // namespace nt2h

@@ -114,9 +114,9 @@ __global__ __launch_bounds__(kBlock) void fold_f32_kernel(const float* __restric
   }
   __shared__ float lds[2 * kWaves];
   Comp c[1] = {acc.finish()};
-  double bsum[1];
+  double bsum[1] = {0.0};
   block_reduce_comp<1>(c, lds, bsum);
-  double res[1];
+  double res[1] = {0.0};
   if (grid_combine<1>(bsum, ws, res)) out[0] = res[0];
 }
 
@@ -142,7 +142,7 @@ __global__ __launch_bounds__(kBlock) void fold_f64_kernel(const double* __restri
     for (int w = 1; w < kWaves; ++w) t += lds[w];
     bsum[0] = t;
   }
-  double res[1];
+  double res[1] = {0.0};
   if (grid_combine<1>(bsum, ws, res)) out[0] = res[0];
 }
 

@@ -1,0 +1,163 @@
+// C++ front-end check: NT2-style table expressions on the GPU through
+// include/nt2hip/table.hpp, compared with the oracle (libnt2ref.so) on the
+// CPU.  Prints one "PASS name" / "FAIL name ..." line per check and exits
+// non-zero on any failure.  Built by tests/cpp/Makefile, run by
+// tests/test_gpu_frontend.py.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "../../include/nt2hip/table.hpp"
+
+extern "C" {
+void nt2ref_map_f32(int op, const float* x, float* y, int64_t n);
+void nt2ref_saxpy_f32(float a, const float* x, float* y, int64_t n);
+void nt2ref_bs_fwd_f32(const float*, const float*, const float*, const float*, const float*, float*, float*, int64_t);
+void nt2ref_exact_dot_f32(const float* x, const float* y, int64_t n, double* out);
+void nt2ref_exact_sum_f32(const float* x, int64_t n, double* out);
+}
+
+using nt2hip::table;
+static int failures = 0;
+static void report(const char* name, bool ok, const char* detail = "") {
+  std::printf("%s %s %s\n", ok ? "PASS" : "FAIL", name, detail);
+  if (!ok) ++failures;
+}
+static uint32_t bits(float f) { uint32_t u; std::memcpy(&u, &f, 4); return u; }
+static bool same_bits(const std::vector<float>& a, const std::vector<float>& b) {
+  if (a.size() != b.size()) return false;
+  for (size_t i = 0; i < a.size(); ++i)
+    if (bits(a[i]) != bits(b[i]) && !(std::isnan(a[i]) && std::isnan(b[i]))) return false;
+  return true;
+}
+static std::vector<float> uniform(size_t n, float lo, float hi, uint64_t seed) {
+  std::vector<float> v(n);
+  uint64_t z = seed;
+  for (auto& x : v) {
+    z += 0x9E3779B97F4A7C15ull;
+    uint64_t t = z;
+    t = (t ^ (t >> 30)) * 0xBF58476D1CE4E5B9ull;
+    t = (t ^ (t >> 27)) * 0x94D049BB133111EBull;
+    t ^= t >> 31;
+    x = (float)(lo + (hi - lo) * ((t >> 11) * 0x1.0p-53));
+  }
+  return v;
+}
+
+int main() {
+  const int64_t n = (1 << 20) + 7;
+  auto hx = uniform(n, -1.f, 1.f, 1), hy = uniform(n, -1.f, 1.f, 2);
+  table<float> X(hx), Y(hy);
+
+  {  // Y = a*X + Y lowers to nt2h_saxpy_f32; bit-exact vs NT2
+    Y = 2.5f * X + Y;
+    std::vector<float> want = hy;
+    nt2ref_saxpy_f32(2.5f, hx.data(), want.data(), n);
+    report("saxpy_lowered_bit_exact", same_bits(Y.to_host(), want));
+    Y = table<float>(hy);
+  }
+  {  // a single functor node lowers to nt2h_map_f32 (EXACT): bit-exact
+    table<float> Z;
+    Z = exp(X);
+    std::vector<float> want(n);
+    nt2ref_map_f32(0, hx.data(), want.data(), n);
+    report("exp_lowered_bit_exact", same_bits(Z.to_host(), want));
+  }
+  {  // an arbitrary tree: one generated kernel, NT2 two-rounding semantics
+    table<float> Z;
+    Z = exp(-sqr(X) / 2.0f) * cos(3.0f * X + Y) + tanh(Y);
+    std::vector<float> e(n), c(n), t(n), arg(n), q(n), want(n);
+    for (int64_t i = 0; i < n; ++i) { q[i] = -(hx[i] * hx[i]) / 2.0f; float a = 3.0f * hx[i]; arg[i] = a + hy[i]; }
+    nt2ref_map_f32(0, q.data(), e.data(), n);
+    nt2ref_map_f32(8, arg.data(), c.data(), n);
+    nt2ref_map_f32(6, hy.data(), t.data(), n);
+    for (int64_t i = 0; i < n; ++i) { float p = e[i] * c[i]; want[i] = p + t[i]; }
+    report("fused_tree_bit_exact", same_bits(Z.to_host(), want));
+  }
+  {  // globalsum / dot lower to the compensated folds
+    double ex[2];
+    nt2ref_exact_dot_f32(hx.data(), hy.data(), n, ex);
+    float d = nt2hip::dot(X, Y);
+    float want = (float)(ex[0] + ex[1]);
+    float sp = std::nextafter(std::fabs(want), INFINITY) - std::fabs(want);
+    char buf[128];
+    std::snprintf(buf, sizeof buf, "got %.9g want %.9g", d, want);
+    report("dot_within_1ulp", std::fabs(d - want) <= sp, buf);
+    nt2ref_exact_sum_f32(hx.data(), n, ex);
+    float s = nt2hip::globalsum(X);
+    want = (float)(ex[0] + ex[1]);
+    sp = std::nextafter(std::fabs(want), INFINITY) - std::fabs(want);
+    report("globalsum_within_1ulp", std::fabs(s - want) <= sp);
+    // a generic fold of a tree
+    float s2 = nt2hip::globalsum(X * X + 1.0f);
+    double ref = 0;
+    for (int64_t i = 0; i < n; ++i) { float p = hx[i] * hx[i]; ref += (double)(p + 1.0f); }
+    std::snprintf(buf, sizeof buf, "got %.9g want %.12g", s2, ref);
+    report("globalsum_tree", std::fabs(s2 - ref) <= 1e-6 * std::fabs(ref), buf);
+    double s3 = nt2hip::globalsum(nt2hip::table<double>(std::vector<double>(1000, 0.5)) * 2.0);
+    std::snprintf(buf, sizeof buf, "got %.17g want 1000", s3);
+    report("globalsum_tree_f64", s3 == 1000.0, buf);
+  }
+  {  // tie: both outputs from one loop nest, inputs read before outputs written
+    table<float> A(hx), B(hy);
+    nt2hip::tie(A, B) = nt2hip::tie(A + B, A - B);
+    auto ha = A.to_host(), hb = B.to_host();
+    bool ok = true;
+    for (int64_t i = 0; i < n && ok; ++i) ok = (ha[i] == hx[i] + hy[i]) && (hb[i] == hx[i] - hy[i]);
+    report("tie_reads_before_writes", ok);
+  }
+  {  // 2-D: sum along dimension 1 (inner_fold) and 2 (outer_fold), sx broadcast
+    const int64_t rows = 1000, cols = 37;
+    auto hm = uniform(rows * cols, -2.f, 2.f, 3), hs = uniform(cols, 0.5f, 2.f, 4), hk = uniform(cols, 0.1f, 10.f, 5);
+    table<float> M(hm, rows, cols), Sx(hs, 1, cols), Kx(hk, 1, cols);
+    table<float> z = nt2hip::sum(exp(-sqr(M) / (2.0f * sqr(nt2hip::sx(Sx)))) * cos(nt2hip::sx(Kx) * M), 1);
+    table<float> zr = nt2hip::sum(M * M, 2);
+    auto hz = z.to_host(), hzr = zr.to_host();
+    bool ok1 = z.rows() == 1 && z.cols() == cols, ok2 = zr.rows() == rows && zr.cols() == 1;
+    for (int64_t j = 0; j < cols && ok1; ++j) {
+      double ref = 0, mag = 0;
+      for (int64_t i = 0; i < rows; ++i) {
+        double x = hm[j * rows + i], e = std::exp(-x * x / (2.0 * hs[j] * hs[j]));
+        ref += e * std::cos((double)hk[j] * x);
+        mag += e;
+      }
+      ok1 = std::fabs(hz[j] - ref) <= 1e-5 * mag;
+    }
+    for (int64_t i = 0; i < rows && ok2; ++i) {
+      double ref = 0;
+      for (int64_t j = 0; j < cols; ++j) ref += (double)hm[j * rows + i] * hm[j * rows + i];
+      ok2 = std::fabs(hzr[i] - ref) <= 1e-6 * ref;
+    }
+    report("colfold_sum_dim1_sx", ok1);
+    report("rowfold_sum_dim2", ok2);
+  }
+  {  // Black-Scholes through tie (5 -> 2 streams): bit-exact vs NT2's scalar composition
+    const int64_t m = 100003;
+    auto hS = uniform(m, 5.f, 30.f, 6), hK = uniform(m, 1.f, 100.f, 7), hT = uniform(m, 0.25f, 10.f, 8);
+    std::vector<float> hr(m, 0.02f), hv(m, 0.30f), wc(m), wp(m);
+    table<float> S(hS), K(hK), T(hT), r(hr), v(hv), call, put;
+    nt2hip::tie(call, put) = nt2hip::blackscholes(S, K, T, r, v);
+    nt2ref_bs_fwd_f32(hS.data(), hK.data(), hT.data(), hr.data(), hv.data(), wc.data(), wp.data(), m);
+    report("blackscholes_tie_bit_exact", same_bits(call.to_host(), wc) && same_bits(put.to_host(), wp));
+    auto hw = uniform(m, -1.f, 1.f, 9);
+    table<float> w(hw), price, dS, dK, dT, dr, dsig;
+    auto tot = nt2hip::blackscholes_adjoint(S, K, T, r, v, w, price, dS, dK, dT, dr, dsig);
+    auto hdr = dr.to_host();
+    double sdr = 0, adr = 0;
+    for (float x : hdr) { sdr += x; adr += std::fabs(x); }
+    report("blackscholes_adjoint_totals", std::fabs(tot.dP_dr - sdr) <= 1e-9 * adr);
+  }
+  {  // errors surface as exceptions
+    bool threw = false;
+    try {
+      table<float> A(10), B(11), C;
+      C = A + B;
+    } catch (const nt2hip::error& e) {
+      threw = e.code == NT2H_ERR_INVALID_ARG;
+    }
+    report("size_mismatch_throws", threw);
+  }
+  std::printf("%d failure(s)\n", failures);
+  return failures ? 1 : 0;
+}
