@@ -42,7 +42,8 @@ def saxpy_(a: float, x, y):
     """y <- a*x + y in place (two roundings, bit-identical to NT2's scalar path)."""
     torch = _torch()
     _chk(x, torch.float32, "x"), _chk(y, torch.float32, "y")
-    assert x.numel() == y.numel()
+    if x.numel() != y.numel():
+        raise ValueError(f"dot: sizes differ ({x.numel()} vs {y.numel()})")
     call("nt2h_saxpy_f32", C.c_float(a), _ptr(x), _ptr(y), y.numel(), _stream())
     return y
 
@@ -91,7 +92,8 @@ def sum(x, mode: int = SUM_KAHAN, out=None):  # noqa: A001 (NT2 vocabulary: sum 
 
 def dot(x, y, mode: int = SUM_KAHAN, out=None):
     torch = _torch()
-    assert x.numel() == y.numel()
+    if x.numel() != y.numel():
+        raise ValueError(f"dot: sizes differ ({x.numel()} vs {y.numel()})")
     if x.dtype == torch.float32:
         return _fold("nt2h_dot_f32", [_chk(x, torch.float32, "x"), _chk(y, torch.float32, "y")], x.numel(),
                      [mode], out)
