@@ -64,6 +64,12 @@ def _cpu_baseline(inputs_np, budget_s=12.0):
         el = time.perf_counter() - t0
         if el >= budget_s or reps >= 2000:
             break
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
     return {
         "value": n * reps / el,
         "unit": "options/s",
@@ -71,14 +77,20 @@ def _cpu_baseline(inputs_np, budget_s=12.0):
         "kind": "port",
         "sample": f"first 2^22 options of the workload, {reps} full passes in {el:.1f} s "
                   f"(NT2 scalar functors restated, OpenMP static grains of L2/4 floats)",
+        "gbps": BYTES_PER_OPTION * n * reps / el / 1e9,
+        "cpu_model": model,
     }
 
 
 def _events(stream, fn, reps):
+    """Seconds per launch over `reps` back-to-back launches, after >= 30 ms of
+    warm-up launches (clocks ramp up after the host-side input generation)."""
     import torch
 
-    fn()
-    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < 0.03:
+        fn()
+        torch.cuda.synchronize()
     a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     a.record(stream)
     for _ in range(reps):
@@ -108,41 +120,78 @@ def _ulp_vs_fp64(y32, y64):
     return float(u.max()), int(u.argmax()), float(rel.max()), int(sub.sum()), sub_abs
 
 
-def report_sweep(dev, stream, n=1 << 28):
+def _rounded_ulps(y32, y64):
+    """Max std-ULP (IEEE spacing steps) and max NT2-ULP (modules/test/unit/src/ulp.cpp:18-32:
+    frexp-aligned difference over float epsilon) of y32 against y64 rounded to float,
+    over the elements where the rounded value is a normal float or zero."""
+    import torch
+
+    r = y64.float()
+    tiny = torch.finfo(torch.float32).tiny
+    ok = ((r.abs() >= tiny) | (r == 0)) & torch.isfinite(r) & torch.isfinite(y32)
+    ok &= (y32.abs() >= tiny) | (y32 == 0)
+
+    def ordv(t):
+        i = t.view(torch.int32).to(torch.int64)
+        return torch.where(i < 0, -(1 << 31) - i, i)
+
+    std = torch.where(ok, (ordv(y32) - ordv(r)).abs(), torch.zeros((), dtype=torch.int64, device=r.device))
+    m1, e1 = torch.frexp(y32)
+    m2, e2 = torch.frexp(r)
+    ex = -torch.maximum(e1, e2)
+    d = torch.where(e1 == e2, (m1 - m2).abs(), (torch.ldexp(y32, ex) - torch.ldexp(r, ex)).abs())
+    nt2 = torch.where(ok & (y32 != r), d / torch.finfo(torch.float32).eps, torch.zeros_like(d))
+    return int(std.max()), float(nt2.max())
+
+
+def report_sweep(dev, stream, n=1 << 28, fns=("exp", "log", "sqrt", "erfc", "tanh", "sin", "cos")):
     """Precision sweep: each functor over 2^28 floats, FP32 (EXACT, FUSED) vs FP64 on the GPU."""
     import torch
 
     from nt2_amd import EXACT, FUSED, SUM_KAHAN, SUM_NAIVE, SUM_PAIRWISE, ops, synth
+    from nt2_amd.layout import streams
 
     lines = []
-    for fn in ("exp", "log", "sqrt", "erfc", "tanh", "sin", "cos"):
-        x = torch.from_numpy(synth.sweep_args(fn, n)).to(dev)
-        x64 = x.double()
-        y64 = ops.map(fn, x64, EXACT)
+    for fn in fns:
+        x, y = streams(2, n, torch.float32, dev)
+        x.copy_(torch.from_numpy(synth.sweep_args(fn, n)))
+        x64, y64 = streams(2, n, torch.float64, dev)
+        x64.copy_(x)
+        ops.map(fn, x64, EXACT, out=y64)
         t64 = _events(stream, lambda: ops.map(fn, x64, EXACT, out=y64), 3)
         rec = {"report": "precision_sweep", "functor": fn, "n": n, "args": synth.SWEEP[fn],
                "f64_us": t64 * 1e6, "f64_gbps": 16 * n / t64 / 1e9}
         for name, pr in (("exact", EXACT), ("fused", FUSED)):
-            y = ops.map(fn, x, pr)
+            ops.map(fn, x, pr, out=y)
             t = _events(stream, lambda: ops.map(fn, x, pr, out=y), 5)
             u, am, rel, nsub, sub_abs = _ulp_vs_fp64(y, y64)
+            std_u, nt2_u = _rounded_ulps(y, y64)
             rec[name] = {"us": t * 1e6, "gbps": 8 * n / t / 1e9, "hbm_frac": 8 * n / t / 8e12,
                          "max_ulp_vs_fp64": u, "worst_arg": float(x[am]), "max_rel_vs_fp64": rel,
+                         "max_std_ulp_vs_fp64_rounded": std_u, "max_nt2_ulp_vs_fp64_rounded": nt2_u,
                          "subnormal_band_count": nsub, "subnormal_band_max_abs_err": sub_abs,
                          "speedup_vs_f64": t64 / t}
-            del y
         lines.append(rec)
-        del x, x64, y64
+        del x, y, x64, y64
         torch.cuda.empty_cache()
     a_np, b_np = synth.sumdot_inputs(n)
-    a, b = torch.from_numpy(a_np).to(dev), torch.from_numpy(b_np).to(dev)
-    a64, b64 = a.double(), b.double()
+    a, b = streams(2, n, torch.float32, dev)
+    a.copy_(torch.from_numpy(a_np))
+    b.copy_(torch.from_numpy(b_np))
+    a64, b64 = streams(2, n, torch.float64, dev)
+    a64.copy_(a)
+    b64.copy_(b)
     s64 = ops.sum(a64).item()
     d64 = ops.dot(a64, b64).item()
     t64s = _events(stream, lambda: ops.sum(a64), 5)
     t64d = _events(stream, lambda: ops.dot(a64, b64), 5)
+    # NT2's scalar inner_fold order (left to right in float), for information
+    seq_s = float(np.cumsum(a_np, dtype=np.float32)[-1])
+    seq_d = float(np.cumsum(a_np * b_np, dtype=np.float32)[-1])
     rec = {"report": "sum_dot", "n": n, "values": "N(1, 1e-3)", "f64": {"sum": s64, "dot": d64,
-           "sum_us": t64s * 1e6, "dot_us": t64d * 1e6}}
+           "sum_us": t64s * 1e6, "dot_us": t64d * 1e6},
+           "nt2_sequential_order": {"sum_ulp_f32": abs(seq_s - s64) / float(np.spacing(np.float32(s64))),
+                                    "dot_ulp_f32": abs(seq_d - d64) / float(np.spacing(np.float32(d64)))}}
     for name, mode in (("naive", SUM_NAIVE), ("kahan", SUM_KAHAN), ("pairwise", SUM_PAIRWISE)):
         s = ops.sum(a, mode).item()
         d = ops.dot(a, b, mode).item()
@@ -162,10 +211,15 @@ def report_bs_fwd(dev, stream, n=1 << 26):
     import torch
 
     from nt2_amd import EXACT, FUSED, ops, synth
+    from nt2_amd.layout import streams
 
     d = synth.bs_inputs(n)
-    g = [torch.from_numpy(d[k]).to(dev) for k in ("S", "K", "T", "r", "sig")]
-    g64 = [t.double() for t in g]
+    g = streams(5, n, torch.float32, dev)
+    for t, k in zip(g, ("S", "K", "T", "r", "sig")):
+        t.copy_(torch.from_numpy(d[k]))
+    g64 = streams(5, n, torch.float64, dev)
+    for t64, t in zip(g64, g):
+        t64.copy_(t)
     c64, p64 = ops.bs_fwd(*g64, precision=EXACT)
     t64 = _events(stream, lambda: ops.bs_fwd(*g64, precision=EXACT), 5)
     rec = {"report": "bs_fwd", "options": n, "f64_us": t64 * 1e6, "f64_gbps": 56 * n / t64 / 1e9}
@@ -194,6 +248,27 @@ def report_saxpy(dev, stream, n=1 << 20):
     return [{"report": "saxpy_sdot", "n": n, "saxpy_us": t * 1e6, "saxpy_gbps": 12 * n / t / 1e9,
              "sdot_us": td * 1e6, "sdot_gbps": 8 * n / td / 1e9,
              "note": "2^20 elements: launch-latency bound, not a bandwidth figure"}]
+
+
+def report_placement(dev, stream, n=1 << 24):
+    """The headline kernel on coloured tables vs 12 separate allocations."""
+    import torch
+
+    from nt2_amd import ops, synth
+    from nt2_amd.layout import streams
+
+    d = synth.bs_inputs(n, with_w=True)
+    host = [torch.from_numpy(d[k]) for k in ("S", "K", "T", "r", "sig", "w")]
+    col = streams(12, n, torch.float32, dev)
+    for t, h in zip(col, host):
+        t.copy_(h)
+    sep = [h.to(dev) for h in host] + [torch.empty(n, dtype=torch.float32, device=dev) for _ in range(6)]
+    rec = {"report": "placement", "options": n}
+    for name, tabs in (("coloured", col), ("separate_allocations", sep)):
+        offs = sorted((t.data_ptr() - tabs[0].data_ptr()) % (2 << 20) for t in tabs)
+        t = _events(stream, lambda: ops.bs_fwd_adj(*tabs[:6], totals=True, outs=tabs[6:]), 50)
+        rec[name] = {"kernel_us": t * 1e6, "gbps": 48 * n / t / 1e9, "offsets_mod_2MiB_KiB": [o // 1024 for o in offs]}
+    return [rec]
 
 
 def report_colfold(dev, stream, rows=16384, cols=8192):
@@ -225,11 +300,13 @@ def report_colfold(dev, stream, rows=16384, cols=8192):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--report", default="", help="extra report lines after the headline: "
-                    "comma list of sweep,bs_fwd,saxpy,colfold (or all)")
+                    "comma list of placement,sweep,bs_fwd,saxpy,colfold (or all)")
+    ap.add_argument("--sweep-fns", default="exp,log,sqrt,erfc,tanh,sin,cos",
+                    help="functors of the precision sweep report")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -241,6 +318,7 @@ def main():
 
     import nt2_amd
     from nt2_amd import ops, synth
+    from nt2_amd.layout import streams
 
     nt2_amd.call("nt2h_init", 0)
     dev = torch.device("cuda:0")
@@ -248,12 +326,16 @@ def main():
     d = synth.bs_inputs(n, with_w=True)
     keys = ("S", "K", "T", "r", "sig", "w")
     host = [d[k] for k in keys]
-    g = [torch.from_numpy(a).to(dev) for a in host]
-    outs = [torch.empty_like(g[0]) for _ in range(6)]
+    # the 12 tables of the loop nest, placed 4 KiB apart modulo 2 MiB
+    tabs = streams(12, n, torch.float32, dev)
+    g, outs = tabs[:6], tabs[6:]
+    for t, a in zip(g, host):
+        t.copy_(torch.from_numpy(a))
+    tot = torch.empty(2, dtype=torch.float64, device=dev)
     stream = torch.cuda.current_stream()
 
     def step():
-        return ops.bs_fwd_adj(*g, totals=True, outs=outs)
+        return ops.bs_fwd_adj(*g, totals=True, outs=outs, tot_out=tot)
 
     for _ in range(args.warmup):
         step()
@@ -276,10 +358,22 @@ def main():
     achieved = BYTES_PER_OPTION * n / avg_kern_s / 1e9
     totals32 = res[6].cpu().numpy()
     f32_out = [o.clone() for o in outs]
+    # per-launch event pairs: median of 50 launches
+    per = []
+    for _ in range(50):
+        p0, p1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        p0.record(stream)
+        step()
+        p1.record(stream)
+        per.append((p0, p1))
+    torch.cuda.synchronize()
+    med_us = float(np.median([a.elapsed_time(b) * 1e3 for a, b in per]))
 
     # plain FP64 evaluation of the same tree on the same (promoted) inputs
-    g64 = [t.double() for t in g]
-    outs64 = [torch.empty_like(g64[0]) for _ in range(6)]
+    tabs64 = streams(12, n, torch.float64, dev)
+    g64, outs64 = tabs64[:6], tabs64[6:]
+    for t64, t in zip(g64, g):
+        t64.copy_(t)
     for _ in range(5):
         r64 = ops.bs_fwd_adj(*g64, totals=True, outs=outs64)
     torch.cuda.synchronize()
@@ -373,7 +467,9 @@ def main():
             "traffic": traffic,
         },
         "kernel_us": avg_kern_s * 1e6,
+        "kernel_us_median_of_50_event_pairs": med_us,
         "hbm_pct_of_peak": 100.0 * achieved / HBM_PEAK_GBPS,
+        "placement": "12 tables in one allocation, each on its own 4 KiB colour (nt2_amd.layout.streams)",
         "copy_gbps_same_bytes": copy_gbps,
         "saxpy_2p26_gbps": saxpy_gbps,
         "fp64": {
@@ -390,13 +486,14 @@ def main():
     print(json.dumps(line), flush=True)
     want = set(args.report.split(",")) if args.report else set()
     if "all" in want:
-        want = {"saxpy", "bs_fwd", "sweep", "colfold"}
-    del g, outs, g64, outs64
+        want = {"saxpy", "bs_fwd", "sweep", "colfold", "placement"}
+    del g, outs, tabs, g64, outs64, tabs64
     torch.cuda.empty_cache()
-    for name, fn in (("saxpy", report_saxpy), ("bs_fwd", report_bs_fwd), ("colfold", report_colfold),
-                     ("sweep", report_sweep)):
+    for name, fn in (("placement", report_placement), ("saxpy", report_saxpy), ("bs_fwd", report_bs_fwd),
+                     ("colfold", report_colfold), ("sweep", report_sweep)):
         if name in want:
-            for rec in fn(dev, stream):
+            recs = fn(dev, stream, fns=tuple(args.sweep_fns.split(","))) if name == "sweep" else fn(dev, stream)
+            for rec in recs:
                 print(json.dumps(rec), flush=True)
             torch.cuda.empty_cache()
 

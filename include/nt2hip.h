@@ -84,6 +84,11 @@ int nt2h_init(int device);              /* select device, create per-device stat
 int nt2h_device_count(int* count);
 const char* nt2h_strerror(int code);
 int nt2h_last_hip_error(void);          /* hipError_t of the last failing HIP call (this thread) */
+/* Device tables.  Allocations of 1 MiB or more are 4 KiB aligned and start at
+ * successive 4 KiB "colours" modulo 64 KiB, so that the tables one loop nest
+ * streams together do not share an offset modulo the 2 MiB allocation
+ * alignment (same HBM channel and bank group).  nt2h_free takes exactly the
+ * pointer nt2h_malloc returned. */
 int nt2h_malloc(void** ptr, size_t bytes);
 int nt2h_free(void* ptr);
 int nt2h_host_alloc(void** ptr, size_t bytes); /* pinned host memory */

@@ -146,9 +146,12 @@ __global__ __launch_bounds__(kBlock) void bs_fwd_adj_f32_kernel(BSAdjArgs a, int
     for (int64_t i = gid; i < nv; i += stride) {
       f32x4 o[6];
       bs_vec4(a, i, o);
-      if (TOTALS) {  // pairwise within the vector, then one compensated add
-        acc[0].add((o[4][0] + o[4][1]) + (o[4][2] + o[4][3]));
-        acc[1].add((o[5][0] + o[5][1]) + (o[5][2] + o[5][3]));
+      if (TOTALS) {  // every option through the compensated add (free: HBM-bound)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          acc[0].add(o[4][e]);
+          acc[1].add(o[5][e]);
+        }
       }
       bs_store4(a, i, o);
     }

@@ -21,6 +21,21 @@ void set_hip_error(hipError_t e) { g_last_hip_error = (int)e; }
 namespace {
 std::mutex g_ws_mutex;
 std::map<std::pair<int, hipStream_t>, Workspace> g_ws;
+
+// HBM colouring of large allocations.  A fused loop nest reads and writes
+// element i of all its tables at once; tables that start at the same offset
+// modulo the allocator's 2 MiB alignment put those accesses on the same HBM
+// channel and bank group (a 6-in/6-out stream pattern lost ~15% of its
+// bandwidth that way).  Allocations of kColorMin bytes or more therefore
+// start at successive 4 KiB colours: colour c = (address / 4 KiB) mod
+// kColors, cycling over calls.  The returned pointer is 4 KiB aligned;
+// nt2h_free maps it back to the allocation.
+constexpr size_t kColorMin = size_t(1) << 20;
+constexpr size_t kColorStep = 4096;
+constexpr unsigned kColors = 16;
+std::mutex g_alloc_mutex;
+std::map<void*, void*> g_colored;  // returned pointer -> hipMalloc base
+unsigned g_next_color = 0;
 }  // namespace
 
 int get_workspace(nt2h_stream_t s, Workspace* ws) {
@@ -113,13 +128,36 @@ int nt2h_malloc(void** ptr, size_t bytes) {
   if (!ptr) return NT2H_ERR_NULL_POINTER;
   *ptr = nullptr;
   if (bytes == 0) return NT2H_OK;
-  NT2H_HIP(hipMalloc(ptr, bytes));
+  if (bytes < kColorMin) {
+    NT2H_HIP(hipMalloc(ptr, bytes));
+    return NT2H_OK;
+  }
+  if (bytes > SIZE_MAX - kColorStep * kColors) return NT2H_ERR_TOO_LARGE;
+  void* base = nullptr;
+  NT2H_HIP(hipMalloc(&base, bytes + kColorStep * kColors));
+  std::lock_guard<std::mutex> lock(g_alloc_mutex);
+  const unsigned want = g_next_color++ % kColors;
+  const uintptr_t b = (uintptr_t)base;
+  const uintptr_t first = (b + kColorStep - 1) / kColorStep * kColorStep;  // 4 KiB aligned start
+  const unsigned have = (unsigned)((first / kColorStep) % kColors);
+  void* p = (void*)(first + (size_t)((want + kColors - have) % kColors) * kColorStep);
+  g_colored[p] = base;
+  *ptr = p;
   return NT2H_OK;
 }
 
 int nt2h_free(void* ptr) {
   if (!ptr) return NT2H_OK;
-  NT2H_HIP(hipFree(ptr));
+  void* base = ptr;
+  {
+    std::lock_guard<std::mutex> lock(g_alloc_mutex);
+    auto it = g_colored.find(ptr);
+    if (it != g_colored.end()) {
+      base = it->second;
+      g_colored.erase(it);
+    }
+  }
+  NT2H_HIP(hipFree(base));
   return NT2H_OK;
 }
 

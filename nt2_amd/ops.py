@@ -11,6 +11,7 @@ from __future__ import annotations
 import ctypes as C
 
 from . import (DEFAULT, EXACT, F64_OPS, FUSED, OPS, SUM_KAHAN, call)
+from .layout import streams
 
 
 def _torch():
@@ -38,12 +39,19 @@ def _chk(t, dtype, name="tensor"):
     return t
 
 
+def _same_size(what, *ts):
+    n = ts[0].numel()
+    for t in ts[1:]:
+        if t.numel() != n:
+            raise ValueError(f"{what}: tables differ in size ({n} vs {t.numel()})")
+
+
 def saxpy_(a: float, x, y):
     """y <- a*x + y in place (two roundings, bit-identical to NT2's scalar path)."""
     torch = _torch()
     _chk(x, torch.float32, "x"), _chk(y, torch.float32, "y")
     if x.numel() != y.numel():
-        raise ValueError(f"dot: sizes differ ({x.numel()} vs {y.numel()})")
+        raise ValueError(f"saxpy_: sizes differ ({x.numel()} vs {y.numel()})")
     call("nt2h_saxpy_f32", C.c_float(a), _ptr(x), _ptr(y), y.numel(), _stream())
     return y
 
@@ -58,6 +66,7 @@ def map(op: str, x, precision: int = EXACT, out=None):  # noqa: A001 (NT2 vocabu
         raise TypeError("map: float32 or float64")
     _chk(x, x.dtype, "x")
     y = torch.empty_like(x) if out is None else _chk(out, x.dtype, "out")
+    _same_size("map", x, y)
     call(fn, code, precision, _ptr(x), _ptr(y), x.numel(), _stream())
     return y
 
@@ -68,7 +77,8 @@ def bs_fwd(S, K, T, r, sig, precision: int = EXACT):
     dt = S.dtype
     for n, t in zip("SKTrs", (S, K, T, r, sig)):
         _chk(t, dt, n)
-    call_, put = torch.empty_like(S), torch.empty_like(S)
+    _same_size("bs_fwd", S, K, T, r, sig)
+    call_, put = streams(2, S.numel(), dt, S.device)
     fn = "nt2h_bs_fwd_f32" if dt == torch.float32 else "nt2h_bs_fwd_f64"
     call(fn, precision, *[_ptr(t) for t in (S, K, T, r, sig, call_, put)], S.numel(), _stream())
     return call_, put
@@ -100,16 +110,28 @@ def dot(x, y, mode: int = SUM_KAHAN, out=None):
     return _fold("nt2h_dot_f64", [_chk(x, torch.float64, "x"), _chk(y, torch.float64, "y")], x.numel(), [], out)
 
 
-def bs_fwd_adj(S, K, T, r, sig, w, totals: bool = True, outs=None):
+def bs_fwd_adj(S, K, T, r, sig, w, totals: bool = True, outs=None, tot_out=None):
     """price and the w-weighted input cotangents (dS, dK, dT, dr, dsig), plus
-    totals = [sum dr, sum dsig] as a float64 device tensor (or None)."""
+    totals = [sum dr, sum dsig] as a float64 device tensor (or None); the
+    kernel writes every output, so ``outs``/``tot_out`` may be uninitialised."""
     torch = _torch()
     dt = S.dtype
     for n, t in zip(("S", "K", "T", "r", "sig", "w"), (S, K, T, r, sig, w)):
         _chk(t, dt, n)
+    _same_size("bs_fwd_adj", S, K, T, r, sig, w)
     if outs is None:
-        outs = [torch.empty_like(S) for _ in range(6)]
-    tot = torch.zeros(2, dtype=torch.float64, device=S.device) if totals else None
+        outs = streams(6, S.numel(), dt, S.device)
+    elif len(outs) != 6:
+        raise ValueError("bs_fwd_adj: outs must hold 6 tables")
+    for n, t in zip(("price", "dS", "dK", "dT", "dr", "dsig"), outs):
+        _chk(t, dt, n)
+    _same_size("bs_fwd_adj", S, *outs)
+    tot = None
+    if totals:
+        tot = torch.empty(2, dtype=torch.float64, device=S.device) if tot_out is None else tot_out
+        _chk(tot, torch.float64, "tot_out")
+        if tot.numel() != 2:
+            raise ValueError("bs_fwd_adj: tot_out must hold 2 values")
     fn = "nt2h_bs_fwd_adj_f32" if dt == torch.float32 else "nt2h_bs_fwd_adj_f64"
     call(fn, *[_ptr(t) for t in (S, K, T, r, sig, w)], *[_ptr(o) for o in outs], _ptr(tot), S.numel(), _stream())
     return (*outs, tot)
@@ -124,6 +146,12 @@ def colfold_gc(X, s, k, p, gz=None, grad: bool = True):
     cols, rows = X.shape
     for n, t in zip("skp", (s, k, p)):
         _chk(t, torch.float32, n)
+        if t.numel() != cols:
+            raise ValueError(f"colfold_gc: {n} must hold one value per column ({cols})")
+    if gz is not None:
+        _chk(gz, torch.float32, "gz")
+        if gz.numel() != cols:
+            raise ValueError(f"colfold_gc: gz must hold one value per column ({cols})")
     z = torch.empty(cols, dtype=torch.float32, device=X.device)
     g = [torch.empty_like(z) for _ in range(3)] if grad else [None, None, None]
     call("nt2h_colfold_gc_f32", _ptr(X), rows, cols, rows, _ptr(s), _ptr(k), _ptr(p), _ptr(gz), _ptr(z),
