@@ -132,6 +132,55 @@ int nt2h_dot_f32(const float* x, const float* y, int64_t n, int mode, double* ou
 int nt2h_sum_f64(const double* x, int64_t n, double* out, nt2h_stream_t s);
 int nt2h_dot_f64(const double* x, const double* y, int64_t n, double* out, nt2h_stream_t s);
 
+/* Other reduction_ functors along one dimension of a contiguous table.
+ * NT2 reshapes a reduction along dimension k to (lo, mid, hi) — the product
+ * of the dimensions before k, dimension k, the product of those after —
+ * and folds over the middle index with inner_fold (lo == 1) or outer_fold
+ * (core/reduction/include/nt2/dsl/functions/reduction/run.hpp:48-93):
+ *   out[i + lo*h] = fold_{m = 0..mid-1} x[i + lo*(m + mid*h)]
+ * sum(x, k), prod(x, k), maximum(x, k), minimum(x, k), asum1, asum2,
+ * norm2 = sqrt(asum2), mean = rec(size) * sum (core/reduction/include/nt2/
+ * core/functions/expr/{asum1,asum2,norm2,mean}.hpp; boost/simd/reduction/
+ * functions/{sum,prod,maximum,minimum}.hpp).  Whole-table globalX(x) is
+ * lo = hi = 1, mid = numel.  out holds lo*hi floats.
+ *  - maximum/minimum are bit-identical to NT2's left-to-right a > b ? a : b
+ *    fold from Maxinit = -FLT_MAX (Valmax = FLT_MAX), NaN and signed-zero
+ *    behaviour included.
+ *  - sum-like folds are FP32-compensated with an FP64 cross-segment combine
+ *    (more accurate than NT2's sequential float order, so not bit-identical);
+ *    prod keeps an FP64 mantissa with a separate exponent, so partial
+ *    products never overflow (NT2's float product can, giving inf).
+ *  - mid == 0 yields the neutral element (mean: NaN, as rec(0)*0). */
+typedef enum nt2h_fold_op {
+  NT2H_FOLD_SUM = 0,
+  NT2H_FOLD_PROD = 1,
+  NT2H_FOLD_MAXIMUM = 2,
+  NT2H_FOLD_MINIMUM = 3,
+  NT2H_FOLD_ASUM1 = 4,
+  NT2H_FOLD_ASUM2 = 5,
+  NT2H_FOLD_NORM2 = 6,
+  NT2H_FOLD_MEAN = 7
+} nt2h_fold_op;
+int nt2h_fold_f32(int op, const float* x, int64_t lo, int64_t mid, int64_t hi, float* out, nt2h_stream_t s);
+
+/* Scans along one dimension (NT2 inner_scan / outer_scan over the middle
+ * index of the same (lo, mid, hi) reshaping; core/swar/include/nt2/core/
+ * functions/scalar/{inner,outer}_scan.hpp, details/scan_step.hpp):
+ *   y[i + lo*(m + mid*h)] = fold_{m' <= m} x[i + lo*(m' + mid*h)]
+ * cumsum (plus, Zero), cumprod (multiplies, One), cummax (max, Minf),
+ * cummin (min, Inf) — boost/simd/swar/functions/{cumsum,cumprod,cummax,
+ * cummin}.hpp.  cummax/cummin are bit-identical to NT2's recurrence (NaN
+ * resets included); cumsum uses compensated float pairs with FP64 carries
+ * and cumprod an FP64 mantissa with a separate exponent (no intermediate
+ * overflow).  y may equal x. */
+typedef enum nt2h_scan_op {
+  NT2H_SCAN_CUMSUM = 0,
+  NT2H_SCAN_CUMPROD = 1,
+  NT2H_SCAN_CUMMAX = 2,
+  NT2H_SCAN_CUMMIN = 3
+} nt2h_scan_op;
+int nt2h_scan_f32(int op, const float* x, int64_t lo, int64_t mid, int64_t hi, float* y, nt2h_stream_t s);
+
 /* ------------------------------------------------------------- adjoints
  * NT2 has no differentiation; these pair a forward loop nest with its
  * hand-written reverse sweep, recomputing forward intermediates in

@@ -26,6 +26,8 @@ F64_OPS = {k: v for k, v in OPS.items() if v <= 9}
 EXACT, FUSED = 0, 1
 SUM_NAIVE, SUM_KAHAN, SUM_PAIRWISE = 0, 1, 2
 H2H, H2D, D2H, D2D, DEFAULT = 0, 1, 2, 3, 4
+FOLD_OPS = {"sum": 0, "prod": 1, "maximum": 2, "minimum": 3, "asum1": 4, "asum2": 5, "norm2": 6, "mean": 7}
+SCAN_OPS = {"cumsum": 0, "cumprod": 1, "cummax": 2, "cummin": 3}
 
 STATUS = {
     0: "NT2H_OK", -1: "NT2H_ERR_INVALID_ARG", -2: "NT2H_ERR_NULL_POINTER", -3: "NT2H_ERR_BAD_OP",
@@ -57,6 +59,8 @@ SIGNATURES = {
     "nt2h_bs_fwd_adj_f32": (_I, [_P] * 13 + [_I64, _P]),
     "nt2h_bs_fwd_adj_f64": (_I, [_P] * 13 + [_I64, _P]),
     "nt2h_colfold_gc_f32": (_I, [_P, _I64, _I64, _I64] + [_P] * 8 + [_P]),
+    "nt2h_fold_f32": (_I, [_I, _P, _I64, _I64, _I64, _P, _P]),
+    "nt2h_scan_f32": (_I, [_I, _P, _I64, _I64, _I64, _P, _P]),
 }
 
 

@@ -1,0 +1,452 @@
+// The other reduction_ functors of NT2 (prod, maximum, minimum, asum1,
+// asum2, norm2, mean, and sum with a float result) over one dimension of a
+// contiguous table.
+//
+// NT2 reshapes a reduction along dimension k into (lo, mid, hi) = (product
+// of the dimensions before k, dimension k, product of those after) and runs
+// inner_fold (lo == 1) or outer_fold (lo > 1) over the middle index
+// (core/reduction/include/nt2/dsl/functions/reduction/run.hpp:48-93;
+// scalar/inner_fold.hpp:37-52, scalar/outer_fold.hpp:22-52): out(i, h) is
+// the left-to-right fold of x(i, m, h), m = 0..mid-1, from the functor's
+// neutral element (details/fold_step.hpp:31-43).
+//
+// Kernels (one or two launches per call):
+//  * run kernel (lo == 1, mid >= 512): one workgroup per (run, segment); the
+//    lanes stride over the segment (16-byte loads when aligned), and the
+//    workgroup combines its lanes in a fixed LDS tree;
+//  * column kernel (lo > 1, or short runs): one lane per (output, segment)
+//    folding sequentially; neighbouring lanes read neighbouring addresses;
+//  * combine kernel: when a run is split into segments, one workgroup per
+//    output combines the segment summaries (each lane a contiguous range of
+//    segments, then a fixed tree: the same order on every call).
+// Arithmetic per functor family:
+//  * sum, asum1, asum2, norm2, mean: FP32 TwoSum-compensated lanes (asum2
+//    adds the exact error of each square), an FP64 combine across segments;
+//    the float total then takes NT2's last steps: sqrt for norm2,
+//    rec(float(mid)) * total for mean (expr/mean.hpp:22-23);
+//  * prod: an FP64 mantissa with a separate integer exponent, renormalised
+//    every four factors, so that no partial product overflows or
+//    underflows; the result is rounded to float once;
+//  * maximum, minimum: NT2's selection a > b ? a : b (a < b ? a : b) folded
+//    from Maxinit = -FLT_MAX (Valmax = FLT_MAX) returns, for a fold with no
+//    NaN, the latest element equal to the extreme value, or the neutral if
+//    it is more extreme; a NaN resets the accumulator, so the result is the
+//    fold of the elements after the last NaN (or that NaN when it is the
+//    last element).  Each lane keeps (index of its last NaN, latest extreme
+//    and its index) — a commutative summary — and the rare case where the
+//    overall extreme lies before the last NaN is finished by rescanning
+//    the tail after that NaN.  Results are bit-identical to NT2's.
+#include <hip/hip_runtime.h>
+
+#include "nt2h_common.hpp"
+
+namespace nt2h {
+
+enum FoldOpId {
+  F_SUM = NT2H_FOLD_SUM,
+  F_PROD = NT2H_FOLD_PROD,
+  F_MAX = NT2H_FOLD_MAXIMUM,
+  F_MIN = NT2H_FOLD_MINIMUM,
+  F_ASUM1 = NT2H_FOLD_ASUM1,
+  F_ASUM2 = NT2H_FOLD_ASUM2,
+  F_NORM2 = NT2H_FOLD_NORM2,
+  F_MEAN = NT2H_FOLD_MEAN
+};
+
+// Table geometry: element m of output o = i + lo*h sits at i + lo*(m + mid*h).
+struct Geo {
+  const float* x;
+  int64_t lo, mid, hi;
+  __device__ __forceinline__ int64_t at(int64_t o, int64_t m) const {
+    if (lo == 1) return o * mid + m;
+    const int64_t h = o / lo, i = o - h * lo;
+    return i + lo * (m + mid * h);
+  }
+};
+
+// Segment summaries live in scratch as 4 doubles each, index o*segs + seg.
+struct Summ {
+  double w[4];
+};
+
+// ------------------------------------------------------------ sum family
+template <int OP> struct FSum {
+  struct L {
+    Comp c;
+  };
+  __device__ static L ident() { return L{{0.0f, 0.0f}}; }
+  __device__ static void add(L& a, float x, int64_t) {
+    if (OP == F_ASUM1) {
+      a.c.add(__builtin_fabsf(x));
+    } else if (OP == F_ASUM2 || OP == F_NORM2) {
+      float ax = __builtin_fabsf(x);
+      float p = ax * ax;
+      float e = __builtin_fmaf(ax, ax, -p);  // exact error of the square
+      a.c.add(p);
+      a.c.c += e;
+    } else {
+      a.c.add(x);
+    }
+  }
+  __device__ static L comb(L a, const L& b) {
+    a.c.merge(b.c.s, b.c.c);
+    return a;
+  }
+  __device__ static Summ summ(const L& a) { return Summ{{(double)a.c.s + (double)a.c.c, 0.0, 0.0, 0.0}}; }
+  __device__ static Summ scomb(Summ a, const Summ& b) {
+    a.w[0] += b.w[0];
+    return a;
+  }
+  __device__ static Summ sident() { return Summ{{0.0, 0.0, 0.0, 0.0}}; }
+  __device__ static float finish(const Summ& a, int64_t mid) {
+    float t = (float)a.w[0];
+    if (OP == F_NORM2) return __builtin_sqrtf(t);
+    if (OP == F_MEAN) {
+      float f = 1.0f / (float)mid;  // NT2: rec(value_type(size)) * sum
+      return f * t;
+    }
+    return t;
+  }
+  static constexpr bool kRescan = false;
+};
+
+// ----------------------------------------------------------- prod family
+struct FProd {
+  struct L {
+    double m;
+    int e;
+    int k;  // factors since the last renormalisation
+  };
+  __device__ static void norm(L& a) {
+    if (a.m != 0.0 && __builtin_isfinite(a.m)) {
+      int k;
+      a.m = __builtin_frexp(a.m, &k);
+      a.e += k;
+      // keep the exponent bounded: beyond these the float result is 0 or inf
+      a.e = a.e > (1 << 28) ? (1 << 28) : (a.e < -(1 << 28) ? -(1 << 28) : a.e);
+    }
+    a.k = 0;
+  }
+  __device__ static L ident() { return L{1.0, 0, 0}; }
+  __device__ static void add(L& a, float x, int64_t) {
+    a.m *= (double)x;
+    if (++a.k == 4) norm(a);  // |x| < 2^128: four factors stay inside double's range
+  }
+  __device__ static L comb(L a, L b) {
+    norm(a);
+    norm(b);
+    a.m *= b.m;
+    a.e += b.e;
+    norm(a);
+    return a;
+  }
+  __device__ static Summ summ(L a) {
+    norm(a);
+    return Summ{{a.m, (double)a.e, 0.0, 0.0}};
+  }
+  __device__ static Summ scomb(Summ a, const Summ& b) {
+    L x{a.w[0], (int)a.w[1], 0}, y{b.w[0], (int)b.w[1], 0};
+    L r = comb(x, y);
+    return Summ{{r.m, (double)r.e, 0.0, 0.0}};
+  }
+  __device__ static Summ sident() { return Summ{{1.0, 0.0, 0.0, 0.0}}; }
+  __device__ static float finish(const Summ& a, int64_t) {
+    double m = a.w[0];
+    int e = (int)a.w[1];
+    if (m == 0.0 || !__builtin_isfinite(m)) return (float)m;
+    // |m| in [0.5, 1): clamp e so that ldexp is exact or saturates the same way
+    if (e > 2000) e = 2000;
+    if (e < -2000) e = -2000;
+    return (float)__builtin_ldexp(m, e);
+  }
+  static constexpr bool kRescan = false;
+};
+
+// -------------------------------------------------- maximum / minimum
+template <bool MAX> struct FExt {
+  struct L {
+    int64_t nan;  // index of the last NaN, -1 if none
+    int64_t j;    // index of the latest extreme, -1 if none
+    float v;
+  };
+  __device__ static bool better(float a, float b) { return MAX ? (a > b) : (a < b); }
+  __device__ static L ident() { return L{-1, -1, 0.0f}; }
+  // elements arrive in increasing index order within a lane
+  __device__ static void add(L& a, float x, int64_t idx) {
+    if (x != x) {
+      a.nan = idx;
+    } else if (a.j < 0 || !better(a.v, x)) {
+      a.v = x;
+      a.j = idx;
+    }
+  }
+  __device__ static L comb(L a, const L& b) {
+    L r;
+    r.nan = a.nan > b.nan ? a.nan : b.nan;
+    bool take_b;
+    if (a.j < 0) take_b = true;
+    else if (b.j < 0) take_b = false;
+    else if (better(a.v, b.v)) take_b = false;
+    else if (better(b.v, a.v)) take_b = true;
+    else take_b = b.j > a.j;  // equal values: the later element wins
+    r.j = take_b ? b.j : a.j;
+    r.v = take_b ? b.v : a.v;
+    return r;
+  }
+  __device__ static Summ summ(const L& a) { return Summ{{(double)a.nan, (double)a.j, (double)a.v, 0.0}}; }
+  __device__ static L unsumm(const Summ& s) { return L{(int64_t)s.w[0], (int64_t)s.w[1], (float)s.w[2]}; }
+  __device__ static Summ scomb(const Summ& a, const Summ& b) { return summ(comb(unsumm(a), unsumm(b))); }
+  __device__ static Summ sident() { return summ(ident()); }
+  static constexpr float neutral() { return MAX ? -0x1.fffffep+127f : 0x1.fffffep+127f; }
+  // no NaN: the latest extreme unless the neutral is more extreme
+  __device__ static float pick(const L& a) {
+    if (a.j < 0) return neutral();
+    return better(neutral(), a.v) ? neutral() : a.v;
+  }
+  static constexpr bool kRescan = true;
+};
+
+template <int OP> struct Family;
+template <> struct Family<F_SUM> { using type = FSum<F_SUM>; };
+template <> struct Family<F_ASUM1> { using type = FSum<F_ASUM1>; };
+template <> struct Family<F_ASUM2> { using type = FSum<F_ASUM2>; };
+template <> struct Family<F_NORM2> { using type = FSum<F_NORM2>; };
+template <> struct Family<F_MEAN> { using type = FSum<F_MEAN>; };
+template <> struct Family<F_PROD> { using type = FProd; };
+template <> struct Family<F_MAX> { using type = FExt<true>; };
+template <> struct Family<F_MIN> { using type = FExt<false>; };
+
+// Fixed-order LDS tree over the workgroup; every thread receives the result.
+template <class T, class Comb>
+__device__ __forceinline__ T block_tree(T v, T* sh, Comb comb) {
+  sh[threadIdx.x] = v;
+  __syncthreads();
+#pragma unroll
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] = comb(sh[threadIdx.x], sh[threadIdx.x + s]);
+    __syncthreads();
+  }
+  T r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+// Finish an extreme-value fold of output o from its summary a; all threads
+// of the workgroup call this together (uniform control flow) and thread 0
+// gets the result.
+template <class F>
+__device__ float ext_finish_block(const Geo& g, int64_t o, typename F::L a, typename F::L* sh) {
+  const int64_t mid = g.mid;
+  if (a.nan < 0) return F::pick(a);
+  if (a.nan == mid - 1) return g.x[g.at(o, a.nan)];
+  if (a.j > a.nan) return a.v;
+  // the extreme lies before the last NaN: fold the tail after it
+  typename F::L t = F::ident();
+  for (int64_t m = a.nan + 1 + threadIdx.x; m < mid; m += kBlock) F::add(t, g.x[g.at(o, m)], m);
+  t = block_tree(t, sh, [](typename F::L p, typename F::L q) { return F::comb(p, q); });
+  return t.v;  // the tail holds no NaN and is not empty
+}
+template <class F>
+__device__ float ext_finish_thread(const Geo& g, int64_t o, const typename F::L& a) {
+  const int64_t mid = g.mid;
+  if (a.nan < 0) return F::pick(a);
+  if (a.nan == mid - 1) return g.x[g.at(o, a.nan)];
+  if (a.j > a.nan) return a.v;
+  typename F::L t = F::ident();
+  for (int64_t m = a.nan + 1; m < mid; ++m) F::add(t, g.x[g.at(o, m)], m);
+  return t.v;
+}
+
+// ------------------------------------------------------------ kernels
+// Run kernel: lo == 1; block b -> (run o = b / segs, segment s = b % segs).
+template <int OP, bool VEC>
+__global__ __launch_bounds__(kBlock) void fold_run_kernel(Geo g, int64_t segs, int64_t seglen, float* out,
+                                                          Summ* summ) {
+  using F = typename Family<OP>::type;
+  using L = typename F::L;
+  __shared__ L sh[kBlock];
+  const int64_t b = blockIdx.x;
+  const int64_t o = b / segs, s = b - o * segs;
+  const int64_t m0 = s * seglen;
+  const int64_t m1 = (m0 + seglen < g.mid) ? m0 + seglen : g.mid;
+  const float* run = g.x + o * g.mid;
+  L a = F::ident();
+  if (VEC) {  // run and segment start are 16-byte aligned, seglen % 4 == 0
+    const int64_t nv = (m1 - m0) / 4;
+    const f32x4* v = reinterpret_cast<const f32x4*>(run + m0);
+    for (int64_t k = threadIdx.x; k < nv; k += kBlock) {
+      f32x4 q = __builtin_nontemporal_load(v + k);
+      const int64_t m = m0 + 4 * k;
+      F::add(a, q[0], m);
+      F::add(a, q[1], m + 1);
+      F::add(a, q[2], m + 2);
+      F::add(a, q[3], m + 3);
+    }
+    for (int64_t m = m0 + 4 * nv + threadIdx.x; m < m1; m += kBlock) F::add(a, run[m], m);
+  } else {
+    for (int64_t m = m0 + threadIdx.x; m < m1; m += kBlock) F::add(a, run[m], m);
+  }
+  a = block_tree(a, sh, [](L p, L q) { return F::comb(p, q); });
+  if (segs > 1) {
+    if (threadIdx.x == 0) summ[o * segs + s] = F::summ(a);
+    return;
+  }
+  float r;
+  if constexpr (F::kRescan) r = ext_finish_block<F>(g, o, a, sh);
+  else r = F::finish(F::summ(a), g.mid);
+  if (threadIdx.x == 0) out[o] = r;
+}
+
+// Column kernel: one lane per (output, segment); consecutive lanes take
+// consecutive outputs of the same segment.
+template <int OP>
+__global__ __launch_bounds__(kBlock) void fold_col_kernel(Geo g, int64_t segs, int64_t seglen, float* out,
+                                                          Summ* summ) {
+  using F = typename Family<OP>::type;
+  using L = typename F::L;
+  const int64_t nout = g.lo * g.hi;
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= nout * segs) return;
+  const int64_t s = t / nout, o = t - s * nout;
+  const int64_t m0 = s * seglen;
+  const int64_t m1 = (m0 + seglen < g.mid) ? m0 + seglen : g.mid;
+  L a = F::ident();
+  for (int64_t m = m0; m < m1; ++m) F::add(a, g.x[g.at(o, m)], m);
+  if (segs > 1) {
+    summ[o * segs + s] = F::summ(a);
+    return;
+  }
+  if constexpr (F::kRescan) out[o] = ext_finish_thread<F>(g, o, a);
+  else out[o] = F::finish(F::summ(a), g.mid);
+}
+
+// Combine kernel: one workgroup per output; lane l takes a contiguous range
+// of segments in order, the lanes then combine in a fixed tree (the sum and
+// product families are deterministic but not sequential in rounding; the
+// extreme-value summaries are order-free).
+template <int OP>
+__global__ __launch_bounds__(kBlock) void fold_combine_kernel(Geo g, int64_t segs, float* out, const Summ* summ) {
+  using F = typename Family<OP>::type;
+  __shared__ Summ sh[kBlock];
+  const int64_t o = blockIdx.x;
+  const int64_t per = (segs + kBlock - 1) / kBlock;
+  const int64_t s0 = threadIdx.x * per;
+  const int64_t s1 = (s0 + per < segs) ? s0 + per : segs;
+  Summ a = F::sident();
+  for (int64_t s = s0; s < s1; ++s) a = F::scomb(a, summ[o * segs + s]);
+  a = block_tree(a, sh, [](Summ p, Summ q) { return F::scomb(p, q); });
+  float r;
+  if constexpr (F::kRescan) {
+    __shared__ typename F::L shl[kBlock];
+    r = ext_finish_block<F>(g, o, F::unsumm(a), shl);
+  } else {
+    r = F::finish(a, g.mid);
+  }
+  if (threadIdx.x == 0) out[o] = r;
+}
+
+// Output for an empty middle extent: NT2 folds nothing from the neutral
+// (mean: rec(0) * 0 = NaN).
+__global__ void fold_fill_kernel(float* out, int64_t n, float v) {
+  int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) out[i] = v;
+}
+
+template <int OP>
+int fold_launch(const float* x, int64_t lo, int64_t mid, int64_t hi, float* out, nt2h_stream_t st) {
+  hipStream_t s = as_hip(st);
+  const int64_t nout = lo * hi;
+  Geo g{x, lo, mid, hi};
+  if (lo == 1 && mid >= 512) {
+    // enough workgroups to fill the chip; segments of at least 2048 elements
+    int64_t segs = (8192 + hi - 1) / hi;
+    int64_t maxsegs = (mid + 2047) / 2048;
+    if (segs > maxsegs) segs = maxsegs;
+    if (segs < 1) segs = 1;
+    int64_t seglen = (mid + segs - 1) / segs;
+    seglen = (seglen + 3) / 4 * 4;
+    segs = (mid + seglen - 1) / seglen;
+    const bool vec = (mid % 4 == 0) && aligned16(x);
+    Summ* summ = nullptr;
+    if (segs > 1) NT2H_TRY(get_scratch(st, (size_t)(nout * segs) * sizeof(Summ), (void**)&summ));
+    const int64_t blocks = nout * segs;
+    if (blocks > 0x7fffffffLL) return NT2H_ERR_TOO_LARGE;
+    if (vec) hipLaunchKernelGGL((fold_run_kernel<OP, true>), dim3((unsigned)blocks), dim3(kBlock), 0, s, g, segs, seglen, out, summ);
+    else hipLaunchKernelGGL((fold_run_kernel<OP, false>), dim3((unsigned)blocks), dim3(kBlock), 0, s, g, segs, seglen, out, summ);
+    NT2H_HIP(hipGetLastError());
+    if (segs > 1) {
+      hipLaunchKernelGGL((fold_combine_kernel<OP>), dim3((unsigned)nout), dim3(kBlock), 0, s, g, segs, out,
+                         (const Summ*)summ);
+      NT2H_HIP(hipGetLastError());
+    }
+    return NT2H_OK;
+  }
+  // column kernel: aim for >= 2^18 lanes; segments of at least 16 elements
+  int64_t segs = ((int64_t)1 << 18) / nout;
+  int64_t maxsegs = (mid + 15) / 16;
+  if (segs > maxsegs) segs = maxsegs;
+  if (segs > 4096) segs = 4096;
+  if (segs < 1) segs = 1;
+  int64_t seglen = (mid + segs - 1) / segs;
+  segs = (mid + seglen - 1) / seglen;
+  Summ* summ = nullptr;
+  if (segs > 1) {
+    if (nout > 0x7fffffffLL) return NT2H_ERR_TOO_LARGE;
+    NT2H_TRY(get_scratch(st, (size_t)(nout * segs) * sizeof(Summ), (void**)&summ));
+  }
+  const int64_t lanes = nout * segs;
+  const int64_t blocks = (lanes + kBlock - 1) / kBlock;
+  if (blocks > 0x7fffffffLL) return NT2H_ERR_TOO_LARGE;
+  hipLaunchKernelGGL((fold_col_kernel<OP>), dim3((unsigned)blocks), dim3(kBlock), 0, s, g, segs, seglen, out, summ);
+  NT2H_HIP(hipGetLastError());
+  if (segs > 1) {
+    hipLaunchKernelGGL((fold_combine_kernel<OP>), dim3((unsigned)nout), dim3(kBlock), 0, s, g, segs, out,
+                       (const Summ*)summ);
+    NT2H_HIP(hipGetLastError());
+  }
+  return NT2H_OK;
+}
+
+inline float fold_neutral_result(int op) {
+  switch (op) {
+    case NT2H_FOLD_PROD: return 1.0f;
+    case NT2H_FOLD_MAXIMUM: return -0x1.fffffep+127f;
+    case NT2H_FOLD_MINIMUM: return 0x1.fffffep+127f;
+    case NT2H_FOLD_MEAN: return __builtin_nanf("");
+    default: return 0.0f;
+  }
+}
+
+}  // namespace nt2h
+
+using namespace nt2h;
+
+extern "C" {
+
+int nt2h_fold_f32(int op, const float* x, int64_t lo, int64_t mid, int64_t hi, float* out, nt2h_stream_t s) {
+  if (op < NT2H_FOLD_SUM || op > NT2H_FOLD_MEAN) return NT2H_ERR_BAD_OP;
+  if (lo < 0 || mid < 0 || hi < 0) return NT2H_ERR_INVALID_ARG;
+  if (lo > 0 && hi > 0 && (mid > INT64_MAX / lo || lo * mid > INT64_MAX / hi)) return NT2H_ERR_TOO_LARGE;
+  const int64_t nout = lo * hi;
+  if (nout == 0) return NT2H_OK;
+  if (!out || (mid > 0 && !x)) return NT2H_ERR_NULL_POINTER;
+  if (mid == 0) {
+    unsigned blocks = (unsigned)((nout + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(fold_fill_kernel, dim3(blocks), dim3(kBlock), 0, as_hip(s), out, nout, fold_neutral_result(op));
+    NT2H_HIP(hipGetLastError());
+    return NT2H_OK;
+  }
+  switch (op) {
+    case NT2H_FOLD_SUM: return fold_launch<F_SUM>(x, lo, mid, hi, out, s);
+    case NT2H_FOLD_PROD: return fold_launch<F_PROD>(x, lo, mid, hi, out, s);
+    case NT2H_FOLD_MAXIMUM: return fold_launch<F_MAX>(x, lo, mid, hi, out, s);
+    case NT2H_FOLD_MINIMUM: return fold_launch<F_MIN>(x, lo, mid, hi, out, s);
+    case NT2H_FOLD_ASUM1: return fold_launch<F_ASUM1>(x, lo, mid, hi, out, s);
+    case NT2H_FOLD_ASUM2: return fold_launch<F_ASUM2>(x, lo, mid, hi, out, s);
+    case NT2H_FOLD_NORM2: return fold_launch<F_NORM2>(x, lo, mid, hi, out, s);
+    case NT2H_FOLD_MEAN: return fold_launch<F_MEAN>(x, lo, mid, hi, out, s);
+  }
+  return NT2H_ERR_BAD_OP;
+}
+
+}  // extern "C"

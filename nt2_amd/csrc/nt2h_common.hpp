@@ -52,6 +52,7 @@ inline unsigned stream_grid(int64_t n, int vec, int64_t max_blocks = 65536) {
 }
 
 int get_workspace(nt2h_stream_t s, Workspace* ws);
+int get_scratch(nt2h_stream_t s, size_t bytes, void** out);
 // Deliver nv doubles computed into dev (device memory) to out (device or
 // host memory); for host memory the stream is synchronised.
 int deliver(nt2h_stream_t s, double* out, const double* dev, int nv, bool out_is_host);

@@ -60,6 +60,39 @@ int get_workspace(nt2h_stream_t s, Workspace* ws) {
   return NT2H_OK;
 }
 
+namespace {
+struct Scratch {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+std::mutex g_scratch_mutex;
+std::map<std::pair<int, hipStream_t>, Scratch> g_scratch;
+}  // namespace
+
+// Grow-only device scratch of one (device, stream) for multi-launch folds
+// and scans (segment summaries).  Calls on one stream are ordered, so a
+// buffer is never in use by two calls at once; growing synchronises the
+// stream before freeing the old buffer.
+int get_scratch(nt2h_stream_t s, size_t bytes, void** out) {
+  int dev = 0;
+  NT2H_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  Scratch& sc = g_scratch[std::make_pair(dev, as_hip(s))];
+  if (sc.bytes < bytes) {
+    if (sc.p) {
+      NT2H_HIP(hipStreamSynchronize(as_hip(s)));
+      NT2H_HIP(hipFree(sc.p));
+      sc.p = nullptr;
+      sc.bytes = 0;
+    }
+    size_t want = bytes < (size_t(1) << 20) ? (size_t(1) << 20) : bytes;
+    NT2H_HIP(hipMalloc(&sc.p, want));
+    sc.bytes = want;
+  }
+  *out = sc.p;
+  return NT2H_OK;
+}
+
 int is_host_pointer(const void* p, bool* host) {
   hipPointerAttribute_t attr;
   hipError_t e = hipPointerGetAttributes(&attr, p);

@@ -1,0 +1,384 @@
+// Scans of NT2 (cumsum, cumprod, cummax, cummin) over one dimension of a
+// contiguous table: cumulative_<tag, bop, neutral> with plus/Zero,
+// multiplies/One, max/Minf, min/Inf (boost/simd/swar/functions/{cumsum,
+// cumprod,cummax,cummin}.hpp) run by inner_scan / outer_scan over the middle
+// index of the (lo, mid, hi) reshaping, with the scalar recurrence
+// summary = bop(summary, x(i, m, h)); y(i, m, h) = summary
+// (core/swar/include/nt2/core/functions/details/scan_step.hpp,
+// scalar/{inner,outer}_scan.hpp).
+//
+// Every scan is an ordered monoid scan, so the association is fixed and the
+// results are the same on every call:
+//  * cummax / cummin: the monoid (contains NaN, NT2 recurrence value) with
+//    combine(L, R) = R if R contains a NaN, else (L.nan, L.acc > R.acc ?
+//    L.acc : R.acc) — associative, and each output equals NT2's recurrence
+//    bit for bit (a NaN resets the running value, ties keep the later
+//    element, as in the scalar a0 > a1 ? a0 : a1);
+//  * cumsum: TwoSum-compensated float pairs inside a tile, FP64 carries
+//    across tiles and segments (more accurate than NT2's sequential float
+//    recurrence, so not bit-identical to it);
+//  * cumprod: an FP64 mantissa with a separate integer exponent, rounded to
+//    float per output (no intermediate overflow).
+// Kernels:
+//  * run kernel (lo == 1, mid >= 512): one workgroup per (run, segment)
+//    walks the segment in tiles of 1024 elements (lane l holds elements
+//    4l..4l+3), scans each tile in the workgroup (wave shuffles, then the
+//    four wave totals through LDS) and carries the tile total forward;
+//  * column kernel (lo > 1, or short runs): one lane per (output, segment)
+//    running the recurrence; neighbouring lanes touch neighbouring addresses;
+//  * long runs split into segments: pass 1 computes each segment's total,
+//    pass 2 the exclusive prefix of the segment totals per run (one
+//    workgroup per run), pass 3 rescans each segment from its prefix.
+#include <hip/hip_runtime.h>
+
+#include "nt2h_common.hpp"
+
+namespace nt2h {
+
+struct SGeo {
+  const float* x;
+  float* y;
+  int64_t lo, mid, hi;
+  __device__ __forceinline__ int64_t at(int64_t o, int64_t m) const {
+    if (lo == 1) return o * mid + m;
+    const int64_t h = o / lo, i = o - h * lo;
+    return i + lo * (m + mid * h);
+  }
+};
+struct SSumm {
+  double w[2];
+};
+
+// ------------------------------------------------------------- cumsum
+struct MSum {
+  struct V {
+    float s, c;
+  };
+  using C = double;
+  __device__ static V ident() { return V{0.0f, 0.0f}; }
+  __device__ static V lift(float x) { return V{x, 0.0f}; }
+  __device__ static V comb(const V& a, const V& b) {
+    float t, e;
+    two_sum(a.s, b.s, t, e);
+    return V{t, (a.c + b.c) + e};
+  }
+  __device__ static V shfl_up(const V& v, int d) { return V{__shfl_up(v.s, d, 64), __shfl_up(v.c, d, 64)}; }
+  __device__ static V shfl(const V& v, int l) { return V{__shfl(v.s, l, 64), __shfl(v.c, l, 64)}; }
+  __device__ static C cident() { return 0.0; }
+  __device__ static C cadd(C c, const V& v) { return c + ((double)v.s + (double)v.c); }
+  __device__ static C ccomb(C a, C b) { return a + b; }
+  __device__ static float fin(C c, const V& v) { return (float)(c + ((double)v.s + (double)v.c)); }
+  __device__ static SSumm pack(C c) { return SSumm{{c, 0.0}}; }
+  __device__ static C unpack(const SSumm& s) { return s.w[0]; }
+};
+
+// ------------------------------------------------------------ cumprod
+struct MProd {
+  struct V {
+    double m;
+    int e;
+  };
+  using C = V;
+  __device__ static V norm(V a) {
+    if (a.m != 0.0 && __builtin_isfinite(a.m)) {
+      int k;
+      a.m = __builtin_frexp(a.m, &k);
+      a.e += k;
+      a.e = a.e > (1 << 28) ? (1 << 28) : (a.e < -(1 << 28) ? -(1 << 28) : a.e);
+    }
+    return a;
+  }
+  __device__ static V ident() { return V{1.0, 0}; }
+  __device__ static V lift(float x) { return norm(V{(double)x, 0}); }
+  __device__ static V comb(const V& a, const V& b) { return norm(V{a.m * b.m, a.e + b.e}); }
+  __device__ static V shfl_up(const V& v, int d) { return V{__shfl_up(v.m, d, 64), __shfl_up(v.e, d, 64)}; }
+  __device__ static V shfl(const V& v, int l) { return V{__shfl(v.m, l, 64), __shfl(v.e, l, 64)}; }
+  __device__ static C cident() { return ident(); }
+  __device__ static C cadd(const C& c, const V& v) { return comb(c, v); }
+  __device__ static C ccomb(const C& a, const C& b) { return comb(a, b); }
+  __device__ static float fin(const C& c, const V& v) {
+    V r = comb(c, v);
+    if (r.m == 0.0 || !__builtin_isfinite(r.m)) return (float)r.m;
+    int e = r.e > 2000 ? 2000 : (r.e < -2000 ? -2000 : r.e);
+    return (float)__builtin_ldexp(r.m, e);
+  }
+  __device__ static SSumm pack(const C& c) { return SSumm{{c.m, (double)c.e}}; }
+  __device__ static C unpack(const SSumm& s) { return C{s.w[0], (int)s.w[1]}; }
+};
+
+// ---------------------------------------------------- cummax / cummin
+template <bool MAX> struct MExt {
+  struct V {
+    int nan;    // the covered elements contain a NaN
+    float acc;  // NT2's recurrence over them, started from the neutral
+  };
+  using C = V;
+  __device__ static float neutral() { return MAX ? -__builtin_inff() : __builtin_inff(); }
+  __device__ static V ident() { return V{0, neutral()}; }
+  __device__ static V lift(float x) { return V{x != x ? 1 : 0, x}; }
+  __device__ static V comb(const V& a, const V& b) {
+    if (b.nan) return b;
+    const bool keep_a = MAX ? (a.acc > b.acc) : (a.acc < b.acc);
+    return V{a.nan, keep_a ? a.acc : b.acc};
+  }
+  __device__ static V shfl_up(const V& v, int d) { return V{__shfl_up(v.nan, d, 64), __shfl_up(v.acc, d, 64)}; }
+  __device__ static V shfl(const V& v, int l) { return V{__shfl(v.nan, l, 64), __shfl(v.acc, l, 64)}; }
+  __device__ static C cident() { return ident(); }
+  __device__ static C cadd(const C& c, const V& v) { return comb(c, v); }
+  __device__ static C ccomb(const C& a, const C& b) { return comb(a, b); }
+  __device__ static float fin(const C& c, const V& v) { return comb(c, v).acc; }
+  __device__ static SSumm pack(const C& c) { return SSumm{{(double)c.nan, (double)c.acc}}; }
+  __device__ static C unpack(const SSumm& s) { return C{(int)s.w[0], (float)s.w[1]}; }
+};
+
+template <int OP> struct SFam;
+template <> struct SFam<NT2H_SCAN_CUMSUM> { using type = MSum; };
+template <> struct SFam<NT2H_SCAN_CUMPROD> { using type = MProd; };
+template <> struct SFam<NT2H_SCAN_CUMMAX> { using type = MExt<true>; };
+template <> struct SFam<NT2H_SCAN_CUMMIN> { using type = MExt<false>; };
+
+// Ordered exclusive scan of one value per lane over the workgroup; returns
+// the exclusive prefix of this lane and the workgroup total in *total.
+// lds holds kWaves values.  All threads must call it.
+template <class M>
+__device__ __forceinline__ typename M::V block_exscan(typename M::V v, typename M::V* lds, typename M::V* total) {
+  using V = typename M::V;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  V inc = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    V o = M::shfl_up(inc, d);
+    if (lane >= d) inc = M::comb(o, inc);
+  }
+  if (lane == 63) lds[wave] = inc;
+  __syncthreads();
+  V wpre = M::ident();
+  V tot = M::ident();
+#pragma unroll
+  for (int w = 0; w < kWaves; ++w) {
+    V t = lds[w];
+    if (w < wave) wpre = M::comb(wpre, t);
+    tot = M::comb(tot, t);
+  }
+  __syncthreads();  // lds is reused by the next call
+  V ex = M::shfl_up(inc, 1);
+  if (lane == 0) ex = M::ident();
+  *total = tot;
+  return M::comb(wpre, ex);
+}
+
+// Run kernel body: scan elements [m0, m1) of run o starting from carry;
+// WRITE = false computes only the total (pass 1 of segmented scans).
+template <class M, bool VEC, bool WRITE>
+__device__ typename M::C run_segment(const SGeo& g, int64_t o, int64_t m0, int64_t m1, typename M::C carry) {
+  using V = typename M::V;
+  __shared__ V lds[kWaves];
+  const float* run = g.x + o * g.mid;
+  float* out = g.y + o * g.mid;
+  if (VEC) {
+    for (int64_t base = m0; base < m1; base += 4 * kBlock) {
+      const int64_t m = base + 4 * (int64_t)threadIdx.x;
+      float e[4];
+      if (m + 3 < m1) {
+        f32x4 q = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(run + m));
+        e[0] = q[0];
+        e[1] = q[1];
+        e[2] = q[2];
+        e[3] = q[3];
+      } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) e[k] = (m + k < m1) ? run[m + k] : 0.0f;
+      }
+      V p[4];
+      V acc = M::ident();
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        V xk = (m + k < m1) ? M::lift(e[k]) : M::ident();
+        acc = (k == 0) ? xk : M::comb(acc, xk);
+        p[k] = acc;
+      }
+      V tot;
+      V ex = block_exscan<M>(acc, lds, &tot);
+      if (WRITE) {
+        float r[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) r[k] = M::fin(carry, M::comb(ex, p[k]));
+        if (m + 3 < m1) {
+          f32x4 q = {r[0], r[1], r[2], r[3]};
+          __builtin_nontemporal_store(q, reinterpret_cast<f32x4*>(out + m));
+        } else {
+#pragma unroll
+          for (int k = 0; k < 4; ++k)
+            if (m + k < m1) out[m + k] = r[k];
+        }
+      }
+      carry = M::cadd(carry, tot);
+    }
+  } else {
+    for (int64_t base = m0; base < m1; base += kBlock) {
+      const int64_t m = base + threadIdx.x;
+      V xk = (m < m1) ? M::lift(run[m]) : M::ident();
+      V tot;
+      V ex = block_exscan<M>(xk, lds, &tot);
+      if (WRITE && m < m1) out[m] = M::fin(carry, M::comb(ex, xk));
+      carry = M::cadd(carry, tot);
+    }
+  }
+  return carry;
+}
+
+// lo == 1: block b -> (run o = b / segs, segment s).  PASS 0: single pass
+// (segs == 1); PASS 1: segment totals; PASS 3: rescan from the prefixes.
+template <int OP, bool VEC, int PASS>
+__global__ __launch_bounds__(kBlock) void scan_run_kernel(SGeo g, int64_t segs, int64_t seglen, SSumm* summ) {
+  using M = typename SFam<OP>::type;
+  const int64_t b = blockIdx.x;
+  const int64_t o = b / segs, s = b - o * segs;
+  const int64_t m0 = s * seglen;
+  const int64_t m1 = (m0 + seglen < g.mid) ? m0 + seglen : g.mid;
+  if (PASS == 1) {
+    typename M::C t = run_segment<M, VEC, false>(g, o, m0, m1, M::cident());
+    if (threadIdx.x == 0) summ[o * segs + s] = M::pack(t);
+  } else {
+    typename M::C c = (PASS == 3) ? M::unpack(summ[o * segs + s]) : M::cident();
+    run_segment<M, VEC, true>(g, o, m0, m1, c);
+  }
+}
+
+// Column kernel: lane per (output, segment), the recurrence run in order.
+// PASS as above.
+template <int OP, int PASS>
+__global__ __launch_bounds__(kBlock) void scan_col_kernel(SGeo g, int64_t segs, int64_t seglen, SSumm* summ) {
+  using M = typename SFam<OP>::type;
+  const int64_t nout = g.lo * g.hi;
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= nout * segs) return;
+  const int64_t s = t / nout, o = t - s * nout;
+  const int64_t m0 = s * seglen;
+  const int64_t m1 = (m0 + seglen < g.mid) ? m0 + seglen : g.mid;
+  typename M::C c = (PASS == 3) ? M::unpack(summ[o * segs + s]) : M::cident();
+  typename M::V acc = M::ident();
+  for (int64_t m = m0; m < m1; ++m) {
+    const int64_t k = g.at(o, m);
+    acc = M::comb(acc, M::lift(g.x[k]));
+    if (PASS != 1) g.y[k] = M::fin(c, acc);
+  }
+  if (PASS == 1) summ[o * segs + s] = M::pack(M::cadd(c, acc));
+}
+
+// Pass 2: per output (one workgroup), replace the segment totals by their
+// exclusive prefixes, in segment order.
+template <int OP>
+__global__ __launch_bounds__(kBlock) void scan_prefix_kernel(int64_t segs, SSumm* summ) {
+  using M = typename SFam<OP>::type;
+  using C = typename M::C;
+  __shared__ SSumm sh[kBlock];
+  const int64_t o = blockIdx.x;
+  SSumm* row = summ + o * segs;
+  const int64_t per = (segs + kBlock - 1) / kBlock;
+  const int64_t s0 = threadIdx.x * per;
+  const int64_t s1 = (s0 + per < segs) ? s0 + per : segs;
+  C a = M::cident();
+  for (int64_t s = s0; s < s1; ++s) a = M::ccomb(a, M::unpack(row[s]));
+  sh[threadIdx.x] = M::pack(a);
+  __syncthreads();
+  // exclusive prefix of the lane totals, sequentially in lane order (one lane)
+  if (threadIdx.x == 0) {
+    C run = M::cident();
+    for (int l = 0; l < kBlock; ++l) {
+      C t = M::unpack(sh[l]);
+      sh[l] = M::pack(run);
+      run = M::ccomb(run, t);
+    }
+  }
+  __syncthreads();
+  C pre = M::unpack(sh[threadIdx.x]);
+  for (int64_t s = s0; s < s1; ++s) {
+    C t = M::unpack(row[s]);
+    row[s] = M::pack(pre);
+    pre = M::ccomb(pre, t);
+  }
+}
+
+template <int OP>
+int scan_launch(const float* x, float* y, int64_t lo, int64_t mid, int64_t hi, nt2h_stream_t st) {
+  hipStream_t s = as_hip(st);
+  const int64_t nout = lo * hi;
+  SGeo g{x, y, lo, mid, hi};
+  if (lo == 1 && mid >= 512) {
+    int64_t segs = (8192 + hi - 1) / hi;
+    int64_t maxsegs = (mid + 8191) / 8192;  // segments of at least 8 tiles
+    if (segs > maxsegs) segs = maxsegs;
+    if (segs < 1) segs = 1;
+    int64_t seglen = (mid + segs - 1) / segs;
+    seglen = (seglen + 4 * kBlock - 1) / (4 * kBlock) * (4 * kBlock);
+    segs = (mid + seglen - 1) / seglen;
+    const bool vec = (mid % 4 == 0) && aligned16(x, y);
+    const int64_t blocks = nout * segs;
+    if (blocks > 0x7fffffffLL) return NT2H_ERR_TOO_LARGE;
+    SSumm* summ = nullptr;
+    if (segs == 1) {
+      if (vec) hipLaunchKernelGGL((scan_run_kernel<OP, true, 0>), dim3((unsigned)blocks), dim3(kBlock), 0, s, g, segs, seglen, summ);
+      else hipLaunchKernelGGL((scan_run_kernel<OP, false, 0>), dim3((unsigned)blocks), dim3(kBlock), 0, s, g, segs, seglen, summ);
+      NT2H_HIP(hipGetLastError());
+      return NT2H_OK;
+    }
+    NT2H_TRY(get_scratch(st, (size_t)blocks * sizeof(SSumm), (void**)&summ));
+    if (vec) hipLaunchKernelGGL((scan_run_kernel<OP, true, 1>), dim3((unsigned)blocks), dim3(kBlock), 0, s, g, segs, seglen, summ);
+    else hipLaunchKernelGGL((scan_run_kernel<OP, false, 1>), dim3((unsigned)blocks), dim3(kBlock), 0, s, g, segs, seglen, summ);
+    NT2H_HIP(hipGetLastError());
+    hipLaunchKernelGGL((scan_prefix_kernel<OP>), dim3((unsigned)nout), dim3(kBlock), 0, s, segs, summ);
+    NT2H_HIP(hipGetLastError());
+    if (vec) hipLaunchKernelGGL((scan_run_kernel<OP, true, 3>), dim3((unsigned)blocks), dim3(kBlock), 0, s, g, segs, seglen, summ);
+    else hipLaunchKernelGGL((scan_run_kernel<OP, false, 3>), dim3((unsigned)blocks), dim3(kBlock), 0, s, g, segs, seglen, summ);
+    NT2H_HIP(hipGetLastError());
+    return NT2H_OK;
+  }
+  int64_t segs = ((int64_t)1 << 18) / nout;
+  int64_t maxsegs = (mid + 63) / 64;
+  if (segs > maxsegs) segs = maxsegs;
+  if (segs > 4096) segs = 4096;
+  if (segs < 1) segs = 1;
+  int64_t seglen = (mid + segs - 1) / segs;
+  segs = (mid + seglen - 1) / seglen;
+  const int64_t lanes = nout * segs;
+  const int64_t blocks = (lanes + kBlock - 1) / kBlock;
+  if (blocks > 0x7fffffffLL || (segs > 1 && nout > 0x7fffffffLL)) return NT2H_ERR_TOO_LARGE;
+  SSumm* summ = nullptr;
+  if (segs == 1) {
+    hipLaunchKernelGGL((scan_col_kernel<OP, 0>), dim3((unsigned)blocks), dim3(kBlock), 0, s, g, segs, seglen, summ);
+    NT2H_HIP(hipGetLastError());
+    return NT2H_OK;
+  }
+  NT2H_TRY(get_scratch(st, (size_t)lanes * sizeof(SSumm), (void**)&summ));
+  hipLaunchKernelGGL((scan_col_kernel<OP, 1>), dim3((unsigned)blocks), dim3(kBlock), 0, s, g, segs, seglen, summ);
+  NT2H_HIP(hipGetLastError());
+  hipLaunchKernelGGL((scan_prefix_kernel<OP>), dim3((unsigned)nout), dim3(kBlock), 0, s, segs, summ);
+  NT2H_HIP(hipGetLastError());
+  hipLaunchKernelGGL((scan_col_kernel<OP, 3>), dim3((unsigned)blocks), dim3(kBlock), 0, s, g, segs, seglen, summ);
+  NT2H_HIP(hipGetLastError());
+  return NT2H_OK;
+}
+
+}  // namespace nt2h
+
+using namespace nt2h;
+
+extern "C" {
+
+int nt2h_scan_f32(int op, const float* x, int64_t lo, int64_t mid, int64_t hi, float* y, nt2h_stream_t s) {
+  if (op < NT2H_SCAN_CUMSUM || op > NT2H_SCAN_CUMMIN) return NT2H_ERR_BAD_OP;
+  if (lo < 0 || mid < 0 || hi < 0) return NT2H_ERR_INVALID_ARG;
+  if (lo > 0 && hi > 0 && (mid > INT64_MAX / lo || lo * mid > INT64_MAX / hi)) return NT2H_ERR_TOO_LARGE;
+  if (lo * mid * hi == 0) return NT2H_OK;
+  if (!x || !y) return NT2H_ERR_NULL_POINTER;
+  switch (op) {
+    case NT2H_SCAN_CUMSUM: return scan_launch<NT2H_SCAN_CUMSUM>(x, y, lo, mid, hi, s);
+    case NT2H_SCAN_CUMPROD: return scan_launch<NT2H_SCAN_CUMPROD>(x, y, lo, mid, hi, s);
+    case NT2H_SCAN_CUMMAX: return scan_launch<NT2H_SCAN_CUMMAX>(x, y, lo, mid, hi, s);
+    case NT2H_SCAN_CUMMIN: return scan_launch<NT2H_SCAN_CUMMIN>(x, y, lo, mid, hi, s);
+  }
+  return NT2H_ERR_BAD_OP;
+}
+
+}  // extern "C"

@@ -10,7 +10,7 @@ from __future__ import annotations
 
 import ctypes as C
 
-from . import (DEFAULT, EXACT, F64_OPS, FUSED, OPS, SUM_KAHAN, call)
+from . import (DEFAULT, EXACT, F64_OPS, FOLD_OPS, FUSED, OPS, SCAN_OPS, SUM_KAHAN, call)
 from .layout import streams
 
 
@@ -159,4 +159,81 @@ def colfold_gc(X, s, k, p, gz=None, grad: bool = True):
     return (z, *g) if grad else z
 
 
-__all__ = ["saxpy_", "map", "bs_fwd", "sum", "dot", "bs_fwd_adj", "colfold_gc", "EXACT", "FUSED", "DEFAULT"]
+def nt2_extents(t) -> list[int]:
+    """NT2 extents (column-major) of a row-major torch tensor: reversed shape."""
+    return list(reversed(t.shape)) or [1]
+
+
+def lmh(extents, dim: int) -> tuple[int, int, int]:
+    """The (lo, mid, hi) reshaping NT2 applies for a reduction or scan along
+    1-based dimension ``dim`` of a table with the given extents."""
+    if dim < 1:
+        raise ValueError("dimensions are 1-based")
+    ext = list(extents) + [1] * max(0, dim - len(extents))
+    lo = 1
+    for e in ext[: dim - 1]:
+        lo *= e
+    hi = 1
+    for e in ext[dim:]:
+        hi *= e
+    return lo, ext[dim - 1], hi
+
+
+def first_non_singleton(extents) -> int:
+    for k, e in enumerate(extents, 1):
+        if e != 1:
+            return k
+    return 1
+
+
+def fold(op: str, x, dim: int | None = None, out=None):
+    """NT2 reduction functor ``op`` (sum, prod, maximum, minimum, asum1, asum2,
+    norm2, mean) of table ``x`` along 1-based NT2 dimension ``dim`` (default:
+    the first non-singleton one).  ``x`` is a contiguous float32 CUDA tensor
+    whose reversed shape is the NT2 extent; the result has extent 1 along dim."""
+    torch = _torch()
+    _chk(x, torch.float32, "x")
+    if op not in FOLD_OPS:
+        raise ValueError(f"fold: unknown functor {op!r}")
+    ext = nt2_extents(x)
+    if dim is None:
+        dim = first_non_singleton(ext)
+    lo, mid, hi = lmh(ext, dim)
+    oext = list(ext) + [1] * max(0, dim - len(ext))
+    oext[dim - 1] = 1
+    shape = tuple(reversed(oext))
+    y = torch.empty(shape, dtype=torch.float32, device=x.device) if out is None else _chk(out, torch.float32, "out")
+    if y.numel() != lo * hi:
+        raise ValueError(f"fold: out must hold {lo * hi} values")
+    call("nt2h_fold_f32", FOLD_OPS[op], _ptr(x), lo, mid, hi, _ptr(y), _stream())
+    return y
+
+
+def global_fold(op: str, x, out=None):
+    """globalX(x): the functor over every element (a 1-element float32 tensor)."""
+    torch = _torch()
+    _chk(x, torch.float32, "x")
+    y = torch.empty(1, dtype=torch.float32, device=x.device) if out is None else _chk(out, torch.float32, "out")
+    call("nt2h_fold_f32", FOLD_OPS[op], _ptr(x), 1, x.numel(), 1, _ptr(y), _stream())
+    return y
+
+
+def scan(op: str, x, dim: int | None = None, out=None):
+    """NT2 scan ``op`` (cumsum, cumprod, cummax, cummin) of table ``x`` along
+    1-based NT2 dimension ``dim`` (default: first non-singleton)."""
+    torch = _torch()
+    _chk(x, torch.float32, "x")
+    if op not in SCAN_OPS:
+        raise ValueError(f"scan: unknown functor {op!r}")
+    ext = nt2_extents(x)
+    if dim is None:
+        dim = first_non_singleton(ext)
+    lo, mid, hi = lmh(ext, dim)
+    y = torch.empty_like(x) if out is None else _chk(out, torch.float32, "out")
+    _same_size("scan", x, y)
+    call("nt2h_scan_f32", SCAN_OPS[op], _ptr(x), lo, mid, hi, _ptr(y), _stream())
+    return y
+
+
+__all__ = ["saxpy_", "map", "bs_fwd", "sum", "dot", "bs_fwd_adj", "colfold_gc", "fold", "global_fold", "scan",
+           "lmh", "EXACT", "FUSED", "DEFAULT"]

@@ -586,4 +586,75 @@ template <class T> inline T fold_dot(const T* x, const T* y, int64_t n) {
   return acc;
 }
 
+// ============================================ other reduction_ functors
+// Each is reduction_<tag, bop, neutral> folded left to right by fold_step:
+//   sum_     plus        Zero     (boost/simd/reduction/functions/sum.hpp:26)
+//   prod_    multiplies  One      (boost/simd/reduction/functions/prod.hpp:26)
+//   maximum_ max_        Maxinit  (boost/simd/reduction/functions/maximum.hpp:30;
+//            Maxinit = -FLT_MAX / -DBL_MAX, constant/constants/maxinit.hpp)
+//   minimum_ min_        Valmax   (boost/simd/reduction/functions/minimum.hpp:29)
+// with the scalar bops a0 > a1 ? a0 : a1 and a0 < a1 ? a0 : a1
+// (arithmetic/functions/scalar/{max,min}.hpp:27).  asum1 = sum(abs(x)),
+// asum2 = sum(sqr_abs(x)) (core/functions/expr/{asum1,asum2}.hpp:24),
+// norm2 = sqrt(asum2(x)) (expr/norm2.hpp), mean = rec(T(size)) * sum(x)
+// (expr/mean.hpp:22-23).
+enum FoldOp { FOLD_SUM = 0, FOLD_PROD, FOLD_MAX, FOLD_MIN, FOLD_ASUM1, FOLD_ASUM2, FOLD_NORM2, FOLD_MEAN };
+template <class T> inline T fold_neutral(int op) {
+  switch (op) {
+    case FOLD_PROD: return T(1);
+    case FOLD_MAX: return -std::numeric_limits<T>::max();
+    case FOLD_MIN: return std::numeric_limits<T>::max();
+    default: return T(0);
+  }
+}
+template <class T> inline T fold_pre(int op, T x) {
+  switch (op) {
+    case FOLD_ASUM1: return std::fabs(x);
+    case FOLD_ASUM2: case FOLD_NORM2: { T a = std::fabs(x); return a * a; }
+    default: return x;
+  }
+}
+template <class T> inline T fold_bop(int op, T acc, T x) {
+  switch (op) {
+    case FOLD_PROD: return acc * x;
+    case FOLD_MAX: return (acc > x) ? acc : x;
+    case FOLD_MIN: return (acc < x) ? acc : x;
+    default: return acc + x;
+  }
+}
+// Fold over the middle index of a contiguous (lo, mid, hi) table: the
+// reshaping NT2's reduction run_assign does before inner_fold (lo == 1) or
+// outer_fold (dsl/functions/reduction/run.hpp:48-93);
+// out[i + lo*h] = fold over m of x[i + lo*(m + mid*h)].
+template <class T> inline void fold_lmh(int op, const T* x, int64_t lo, int64_t mid, int64_t hi, T* out) {
+  for (int64_t h = 0; h < hi; ++h)
+    for (int64_t i = 0; i < lo; ++i) {
+      T acc = fold_neutral<T>(op);
+      for (int64_t m = 0; m < mid; ++m) acc = fold_bop(op, acc, fold_pre(op, x[i + lo * (m + mid * h)]));
+      if (op == FOLD_NORM2) acc = std::sqrt(acc);
+      if (op == FOLD_MEAN) { T f = T(1) / T(mid); acc = f * acc; }
+      out[i + lo * h] = acc;
+    }
+}
+// Scans (cumulative_<tag, bop, neutral>, boost/simd/swar/functions/
+// {cumsum,cumprod,cummax,cummin}.hpp: Zero, One, Minf, Inf) over the middle
+// index, the scalar scan_step recurrence summary = bop(summary, in(i));
+// out(i) = summary (core/functions/details/scan_step.hpp, scalar/
+// {inner,outer}_scan.hpp).
+enum ScanOp { SCAN_SUM = 0, SCAN_PROD, SCAN_MAX, SCAN_MIN };
+template <class T> inline void scan_lmh(int op, const T* x, int64_t lo, int64_t mid, int64_t hi, T* y) {
+  const T inf = std::numeric_limits<T>::infinity();
+  for (int64_t h = 0; h < hi; ++h)
+    for (int64_t i = 0; i < lo; ++i) {
+      T acc = op == SCAN_PROD ? T(1) : op == SCAN_MAX ? -inf : op == SCAN_MIN ? inf : T(0);
+      for (int64_t m = 0; m < mid; ++m) {
+        const int64_t k = i + lo * (m + mid * h);
+        T v = x[k];
+        acc = op == SCAN_PROD ? acc * v : op == SCAN_MAX ? ((acc > v) ? acc : v)
+            : op == SCAN_MIN ? ((acc < v) ? acc : v) : acc + v;
+        y[k] = acc;
+      }
+    }
+}
+
 }  // namespace nt2ref

@@ -68,6 +68,8 @@ def lib() -> C.CDLL:
         L.nt2ref_nt2_ulpdist_f64.restype = C.c_double
         L.nt2ref_exhaustive_hist_f32.argtypes = [C.c_int, C.c_float, C.c_float, P]
         L.nt2ref_max_threads.restype = C.c_int
+        for nm in ("nt2ref_fold_lmh_f32", "nt2ref_fold_lmh_f64", "nt2ref_scan_lmh_f32", "nt2ref_scan_lmh_f64"):
+            getattr(L, nm).argtypes = [C.c_int, P, i64, i64, i64, P]
         _lib = L
     return _lib
 
@@ -109,6 +111,33 @@ def saxpy_f32(a: float, x, y) -> np.ndarray:
 def fold_sum_f32(x) -> np.float32:
     x = _f32(x)
     return np.float32(lib().nt2ref_fold_sum_f32(_p(x), x.size))
+
+
+FOLD_OPS = {"sum": 0, "prod": 1, "maximum": 2, "minimum": 3, "asum1": 4, "asum2": 5, "norm2": 6, "mean": 7}
+SCAN_OPS = {"cumsum": 0, "cumprod": 1, "cummax": 2, "cummin": 3}
+
+
+def fold_lmh(op: str, x, lo: int, mid: int, hi: int) -> np.ndarray:
+    """NT2's sequential fold over the middle index of a contiguous (lo, mid, hi)
+    table; returns lo*hi values (index i + lo*h)."""
+    f64 = np.asarray(x).dtype == np.float64
+    x = _f64(x) if f64 else _f32(x)
+    assert x.size == lo * mid * hi
+    out = np.empty(lo * hi, x.dtype)
+    fn = lib().nt2ref_fold_lmh_f64 if f64 else lib().nt2ref_fold_lmh_f32
+    fn(FOLD_OPS[op], _p(x), lo, mid, hi, _p(out))
+    return out
+
+
+def scan_lmh(op: str, x, lo: int, mid: int, hi: int) -> np.ndarray:
+    """NT2's sequential scan (scan_step) over the middle index of a (lo, mid, hi) table."""
+    f64 = np.asarray(x).dtype == np.float64
+    x = _f64(x) if f64 else _f32(x)
+    assert x.size == lo * mid * hi
+    y = np.empty(x.size, x.dtype)
+    fn = lib().nt2ref_scan_lmh_f64 if f64 else lib().nt2ref_scan_lmh_f32
+    fn(SCAN_OPS[op], _p(x), lo, mid, hi, _p(y))
+    return y
 
 
 def fold_dot_f32(x, y) -> np.float32:

@@ -97,6 +97,18 @@ void nt2ref_saxpy_f32(float a, const float* x, float* y, int64_t n) {
 
 // NT2 scalar inner_fold order (fold_step.hpp:39-40): sequential float.
 float nt2ref_fold_sum_f32(const float* x, int64_t n) { return fold_sum(x, n); }
+void nt2ref_fold_lmh_f32(int op, const float* x, int64_t lo, int64_t mid, int64_t hi, float* out) {
+  fold_lmh(op, x, lo, mid, hi, out);
+}
+void nt2ref_fold_lmh_f64(int op, const double* x, int64_t lo, int64_t mid, int64_t hi, double* out) {
+  fold_lmh(op, x, lo, mid, hi, out);
+}
+void nt2ref_scan_lmh_f32(int op, const float* x, int64_t lo, int64_t mid, int64_t hi, float* y) {
+  scan_lmh(op, x, lo, mid, hi, y);
+}
+void nt2ref_scan_lmh_f64(int op, const double* x, int64_t lo, int64_t mid, int64_t hi, double* y) {
+  scan_lmh(op, x, lo, mid, hi, y);
+}
 float nt2ref_fold_dot_f32(const float* x, const float* y, int64_t n) { return fold_dot(x, y, n); }
 double nt2ref_fold_sum_f64(const double* x, int64_t n) { return fold_sum(x, n); }
 double nt2ref_fold_dot_f64(const double* x, const double* y, int64_t n) { return fold_dot(x, y, n); }

@@ -85,6 +85,22 @@ def test_colfold_argument_checks():
     assert L.nt2h_colfold_gc_f32(f, 10, 0, 10, f, f, f, None, f, None, None, None, None) == 0  # no columns
 
 
+def test_fold_scan_argument_checks():
+    L = nt2_amd.lib()
+    f = fake()
+    assert L.nt2h_fold_f32(8, f, 1, 4, 1, f, None) == -3  # unknown functor
+    assert L.nt2h_fold_f32(-1, f, 1, 4, 1, f, None) == -3
+    assert L.nt2h_fold_f32(0, f, -1, 4, 1, f, None) == -1
+    assert L.nt2h_fold_f32(0, None, 1, 4, 1, f, None) == -2
+    assert L.nt2h_fold_f32(0, f, 1, 4, 1, None, None) == -2
+    assert L.nt2h_fold_f32(0, None, 0, 4, 1, None, None) == 0  # no outputs
+    assert L.nt2h_fold_f32(0, f, 1 << 40, 1 << 40, 1, f, None) == -7  # size overflow
+    assert L.nt2h_scan_f32(4, f, 1, 4, 1, f, None) == -3
+    assert L.nt2h_scan_f32(0, f, 1, -4, 1, f, None) == -1
+    assert L.nt2h_scan_f32(0, f, 1, 4, 1, None, None) == -2
+    assert L.nt2h_scan_f32(0, None, 1, 0, 1, None, None) == 0  # empty
+
+
 def test_runtime_argument_checks():
     L = nt2_amd.lib()
     assert L.nt2h_malloc(None, 16) == -2

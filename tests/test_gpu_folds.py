@@ -1,0 +1,252 @@
+"""GPU parity of the other reduction functors and the scans (nt2h_fold_f32,
+nt2h_scan_f32) against the oracle's sequential NT2 restatement.
+
+Bars: maximum/minimum/cummax/cummin bit-identical to NT2 (NaN, signed zero
+and neutrals included); the compensated sum family within 1 float ULP of the
+exactly rounded FP64 result and inside NT2's own sequential error; prod and
+cumprod within 2 ULP of the FP64 product; the NT2 unit vectors exactly.
+"""
+import numpy as np
+import pytest
+
+import nt2_amd
+from nt2_amd import ops
+
+pytestmark = pytest.mark.gpu
+
+# (lo, mid, hi): run kernel single segment, run kernel segmented, short runs
+# (column kernel), outer folds with and without segments, NT2's 3x5x2 table
+SHAPES = [(1, 5000, 37), (1, (1 << 20) + 3, 1), (1, 1 << 18, 2), (1, 100, 1000), (1, 700, 3), (7, 3000, 5),
+          (1000, 3, 7), (3, 5, 2), (2, 40000, 1), (1, 1, 9)]
+
+
+def _dev_tensor(a, dev):
+    import torch
+
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+
+def _fold(op, x, lo, mid, hi, dev):
+    import torch
+
+    gx = _dev_tensor(x, dev)
+    out = torch.empty(lo * hi, dtype=torch.float32, device=dev)
+    nt2_amd.call("nt2h_fold_f32", nt2_amd.FOLD_OPS[op], gx.data_ptr(), lo, mid, hi, out.data_ptr(),
+                 torch.cuda.current_stream().cuda_stream)
+    return out.cpu().numpy()
+
+
+def _scan(op, x, lo, mid, hi, dev):
+    import torch
+
+    gx = _dev_tensor(x, dev)
+    y = torch.empty_like(gx)
+    nt2_amd.call("nt2h_scan_f32", nt2_amd.SCAN_OPS[op], gx.data_ptr(), lo, mid, hi, y.data_ptr(),
+                 torch.cuda.current_stream().cuda_stream)
+    return y.cpu().numpy()
+
+
+def _specials_data(n, seed):
+    """Random values with NaN, +-0, +-inf and +-FLT_MAX sprinkled in."""
+    rng = np.random.default_rng(seed)
+    x = rng.standard_normal(n).astype(np.float32)
+    sp = np.array([np.nan, 0.0, -0.0, np.inf, -np.inf, 3.4028235e38, -3.4028235e38], np.float32)
+    k = rng.random(n) < 0.02
+    x[k] = sp[rng.integers(0, sp.size, int(k.sum()))]
+    return x
+
+
+def _bits_eq(a, b):
+    return (a.view(np.uint32) == b.view(np.uint32)) | (np.isnan(a) & np.isnan(b))
+
+
+@pytest.mark.parametrize("op", ["maximum", "minimum"])
+@pytest.mark.parametrize("shape", SHAPES)
+def test_extreme_folds_bit_identical(dev, ref, op, shape):
+    lo, mid, hi = shape
+    for seed, data in ((1, "specials"), (2, "normal"), (3, "few-values")):
+        if data == "specials":
+            x = _specials_data(lo * mid * hi, seed)
+        elif data == "normal":
+            x = np.random.default_rng(seed).standard_normal(lo * mid * hi).astype(np.float32)
+        else:  # many ties, signed zeros
+            x = np.random.default_rng(seed).choice(np.array([0.0, -0.0, 1.0, -1.0], np.float32), lo * mid * hi)
+        got = _fold(op, x, lo, mid, hi, dev)
+        want = ref.fold_lmh(op, x, lo, mid, hi)
+        eq = _bits_eq(got, want)
+        assert eq.all(), (data, np.nonzero(~eq)[0][:5], got[~eq][:5], want[~eq][:5])
+
+
+def test_extreme_fold_nan_before_the_tail(dev, ref):
+    """The overall maximum before the last NaN: the tail is rescanned."""
+    for n in (1 << 20, 5000, 100):
+        x = np.random.default_rng(4).random(n).astype(np.float32)
+        x[3] = 7.0  # the largest value, early
+        x[n // 2] = np.nan  # the last NaN, after it
+        for op in ("maximum", "minimum"):
+            for lo, mid, hi in ((1, n, 1), (n, 1, 1)):
+                got = _fold(op, x, lo, mid, hi, dev)
+                want = ref.fold_lmh(op, x, lo, mid, hi)
+                assert _bits_eq(got, want).all()
+        x[-1] = np.nan
+        assert np.isnan(_fold("maximum", x, 1, n, 1, dev)[0])
+    # one workgroup per run (single segment): each run has its extreme before a NaN
+    lo, mid, hi = 1, 600, 20
+    x = np.random.default_rng(12).random(lo * mid * hi).astype(np.float32)
+    for h in range(hi):
+        x[h * mid + h] = 5.0
+        x[h * mid + 300 + h] = np.nan
+    x[3 * mid:4 * mid] = np.nan  # an all-NaN run
+    for op in ("maximum", "minimum"):
+        got = _fold(op, x, lo, mid, hi, dev)
+        assert _bits_eq(got, ref.fold_lmh(op, x, lo, mid, hi)).all()
+
+
+def _exact_sums(x, lo, mid, hi, kind):
+    v = x.astype(np.float64).reshape(hi, mid, lo)  # C order of (h, m, i)
+    if kind == "asum1":
+        v = np.abs(v)
+    elif kind in ("asum2", "norm2"):
+        v = v * v
+    return v.sum(axis=1).reshape(-1)  # index i + lo*h
+
+
+@pytest.mark.parametrize("op", ["sum", "asum1", "asum2", "norm2", "mean"])
+@pytest.mark.parametrize("shape", SHAPES)
+def test_sum_family_accuracy(dev, ref, op, shape):
+    lo, mid, hi = shape
+    x = np.random.default_rng(5).standard_normal(lo * mid * hi).astype(np.float32) + np.float32(0.25)
+    got = _fold(op, x, lo, mid, hi, dev)
+    tot = _exact_sums(x, lo, mid, hi, op)
+    t32 = tot.astype(np.float32)
+    if op == "norm2":
+        want = np.sqrt(t32)
+    elif op == "mean":
+        want = (np.float32(1) / np.float32(mid)) * t32  # rec(size) * sum, as NT2
+    else:
+        want = t32
+    # within 1 ULP of NT2's last steps applied to the exactly rounded total
+    ulp = np.spacing(np.abs(want).astype(np.float32))
+    mag = _exact_sums(np.abs(x), lo, mid, hi, "asum1" if op != "asum2" and op != "norm2" else op)
+    assert np.all(np.abs(got.astype(np.float64) - want) <= ulp + 1e-14 * mag), op
+    # and NT2's sequential float fold is no closer than its own error bound
+    seq = ref.fold_lmh(op, x, lo, mid, hi)
+    gamma = mid * np.finfo(np.float32).eps
+    scale = mag * (1.0 / mid if op == "mean" else 1.0)
+    if op == "norm2":
+        scale = np.sqrt(mag)
+    assert np.all(np.abs(seq.astype(np.float64) - got) <= gamma * scale + 2 * ulp)
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_prod(dev, ref, shape):
+    lo, mid, hi = shape
+    rng = np.random.default_rng(6)
+    x = rng.uniform(0.9, 1.1, lo * mid * hi).astype(np.float32)
+    got = _fold("prod", x, lo, mid, hi, dev)
+    want = np.prod(x.astype(np.float64).reshape(hi, mid, lo), axis=1).reshape(-1)
+    assert np.all(np.abs(got - want) <= 2 * np.spacing(want.astype(np.float32)))
+    # exact powers of two: identical to NT2's sequential product
+    p = rng.choice(np.array([0.5, 2.0, -2.0, 1.0], np.float32), lo * mid * hi)
+    if mid <= 100:
+        assert np.array_equal(_fold("prod", p, lo, mid, hi, dev), ref.fold_lmh("prod", p, lo, mid, hi))
+
+
+def test_prod_specials_and_range(dev, ref):
+    x = np.array([2.0] * 200 + [0.5] * 200, np.float32)  # 2^200 mid-way: NT2's float product is inf
+    assert _fold("prod", x, 1, 400, 1, dev)[0] == 1.0
+    assert np.isinf(ref.fold_lmh("prod", x, 1, 400, 1)[0])  # the documented difference
+    for v in ([0.0, np.inf, 1.0], [np.inf, 0.0], [np.nan, 1.0]):
+        a = np.array(v, np.float32)
+        assert np.isnan(_fold("prod", a, 1, a.size, 1, dev)[0])
+    z = np.array([-1.0, 0.0, 3.0], np.float32)
+    g = _fold("prod", z, 1, 3, 1, dev)[0]
+    assert g == 0 and np.signbit(g) == np.signbit(ref.fold_lmh("prod", z, 1, 3, 1)[0])
+
+
+def test_empty_middle_gives_neutrals(dev):
+    import torch
+
+    out = torch.full((4,), 7.0, device=dev)
+    want = {"sum": 0.0, "prod": 1.0, "maximum": -3.4028235e38, "minimum": 3.4028235e38, "norm2": 0.0}
+    for op, v in want.items():
+        nt2_amd.call("nt2h_fold_f32", nt2_amd.FOLD_OPS[op], 0, 2, 0, 2, out.data_ptr(),
+                     torch.cuda.current_stream().cuda_stream)
+        assert np.all(out.cpu().numpy() == np.float32(v)), op
+    nt2_amd.call("nt2h_fold_f32", nt2_amd.FOLD_OPS["mean"], 0, 2, 0, 2, out.data_ptr(),
+                 torch.cuda.current_stream().cuda_stream)
+    assert np.all(np.isnan(out.cpu().numpy()))
+
+
+def test_fold_unit_tables(dev):
+    """NT2 unit tables through the torch-facing wrapper (reversed shapes)."""
+    import torch
+
+    y = torch.tensor([[i + 10 * j for i in range(1, 6)] for j in range(1, 4)], dtype=torch.float32, device=dev)
+    assert ops.fold("maximum", y, 1).cpu().numpy().ravel().tolist() == [15, 25, 35]
+    assert ops.fold("maximum", y, 2).cpu().numpy().ravel().tolist() == [31, 32, 33, 34, 35]
+    assert ops.global_fold("maximum", y).item() == 35
+    a = torch.arange(1, 5, dtype=torch.float32, device=dev)
+    assert ops.global_fold("prod", a).item() == 24
+
+
+# ------------------------------------------------------------------- scans
+@pytest.mark.parametrize("op", ["cummax", "cummin"])
+@pytest.mark.parametrize("shape", SHAPES)
+def test_extreme_scans_bit_identical(dev, ref, op, shape):
+    lo, mid, hi = shape
+    for seed in (7, 8):
+        x = _specials_data(lo * mid * hi, seed)
+        got = _scan(op, x, lo, mid, hi, dev)
+        want = ref.scan_lmh(op, x, lo, mid, hi)
+        eq = _bits_eq(got, want)
+        assert eq.all(), (np.nonzero(~eq)[0][:5], got[~eq][:5], want[~eq][:5])
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_cumsum_accuracy(dev, ref, shape):
+    lo, mid, hi = shape
+    rng = np.random.default_rng(9)
+    x = rng.random(lo * mid * hi).astype(np.float32)  # positive: relative bound
+    got = _scan("cumsum", x, lo, mid, hi, dev)
+    want = np.cumsum(x.astype(np.float64).reshape(hi, mid, lo), axis=1).reshape(-1)
+    w32 = want.astype(np.float32)
+    assert np.all(np.abs(got - want) <= np.spacing(w32)), np.max(np.abs(got - want) / np.spacing(w32))
+    # mixed signs: absolute bound against the running sum of magnitudes
+    z = rng.standard_normal(lo * mid * hi).astype(np.float32)
+    gz = _scan("cumsum", z, lo, mid, hi, dev)
+    wz = np.cumsum(z.astype(np.float64).reshape(hi, mid, lo), axis=1).reshape(-1)
+    mag = np.cumsum(np.abs(z.astype(np.float64)).reshape(hi, mid, lo), axis=1).reshape(-1)
+    assert np.all(np.abs(gz - wz) <= np.spacing(np.abs(wz).astype(np.float32)) + 1e-12 * mag)
+    # NT2's sequential float recurrence stays inside its own error bound of ours
+    seq = ref.scan_lmh("cumsum", z, lo, mid, hi)
+    assert np.all(np.abs(seq - gz) <= mid * np.finfo(np.float32).eps * mag + np.spacing(np.abs(wz).astype(np.float32)))
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_cumprod_accuracy(dev, shape):
+    lo, mid, hi = shape
+    x = np.random.default_rng(10).uniform(0.99, 1.01, lo * mid * hi).astype(np.float32)
+    got = _scan("cumprod", x, lo, mid, hi, dev)
+    want = np.cumprod(x.astype(np.float64).reshape(hi, mid, lo), axis=1).reshape(-1)
+    assert np.all(np.abs(got - want) <= 2 * np.spacing(want.astype(np.float32)))
+
+
+def test_scan_unit_vectors(dev, ref):
+    """modules/core/swar/unit/table/cumsum.cpp: reshape(_(1, 30), 3, 5, 2) along 1 and 2."""
+    a = np.arange(1, 31, dtype=np.float32)
+    assert np.array_equal(_scan("cumsum", a, 1, 3, 10, dev), ref.scan_lmh("cumsum", a, 1, 3, 10))
+    assert np.array_equal(_scan("cumsum", a, 3, 5, 2, dev), ref.scan_lmh("cumsum", a, 3, 5, 2))
+    assert _scan("cumsum", a, 3, 5, 2, dev)[-1] == 120
+    p = np.full(6, 2.0, np.float32)
+    assert np.array_equal(_scan("cumprod", p, 1, 6, 1, dev), 2.0 ** np.arange(1, 7))
+
+
+def test_scan_in_place(dev, ref):
+    import torch
+
+    x = _specials_data(70000, 11)
+    g = _dev_tensor(x, dev)
+    nt2_amd.call("nt2h_scan_f32", nt2_amd.SCAN_OPS["cummax"], g.data_ptr(), 1, 70000, 1, g.data_ptr(),
+                 torch.cuda.current_stream().cuda_stream)
+    assert _bits_eq(g.cpu().numpy(), ref.scan_lmh("cummax", x, 1, 70000, 1)).all()
