@@ -271,6 +271,37 @@ def report_placement(dev, stream, n=1 << 24):
     return [rec]
 
 
+def report_folds(dev, stream):
+    """The other reduction functors and the scans: whole-table over 2^28
+    floats and along each dimension of a 16384 x 8192 table (decimal GB/s of
+    algorithmic traffic: 4 B/element for a fold, 8 B/element for a scan)."""
+    import torch
+
+    from nt2_amd import FOLD_OPS, SCAN_OPS, call
+    from nt2_amd.layout import streams
+
+    lines = []
+    st = torch.cuda.current_stream().cuda_stream
+    n = 1 << 28
+    x, y = streams(2, n, torch.float32, dev)
+    x.normal_()
+    cases = [("global 2^28", 1, n, 1), ("dim1 16384x8192", 1, 16384, 8192), ("dim2 16384x8192", 16384, 8192, 1)]
+    for name, lo, mid, hi in cases:
+        rec = {"report": "folds", "shape": name, "lo_mid_hi": [lo, mid, hi]}
+        out = torch.empty(lo * hi, dtype=torch.float32, device=dev)
+        for op in ("sum", "maximum", "prod", "norm2", "mean"):
+            t = _events(stream, lambda: call("nt2h_fold_f32", FOLD_OPS[op], x.data_ptr(), lo, mid, hi,
+                                             out.data_ptr(), st), 10)
+            rec[op] = {"us": t * 1e6, "gbps": 4 * lo * mid * hi / t / 1e9}
+        for op in ("cumsum", "cummax", "cumprod"):
+            t = _events(stream, lambda: call("nt2h_scan_f32", SCAN_OPS[op], x.data_ptr(), lo, mid, hi,
+                                             y.data_ptr(), st), 10)
+            rec[op] = {"us": t * 1e6, "gbps": 8 * lo * mid * hi / t / 1e9}
+        lines.append(rec)
+    del x, y
+    return lines
+
+
 def report_colfold(dev, stream, rows=16384, cols=8192):
     """Column fold + adjoint over 16384 x 8192: time, and error vs a torch FP64 evaluation on 64 columns."""
     import torch
@@ -304,7 +335,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--report", default="", help="extra report lines after the headline: "
-                    "comma list of placement,sweep,bs_fwd,saxpy,colfold (or all)")
+                    "comma list of placement,sweep,bs_fwd,saxpy,colfold,folds (or all)")
     ap.add_argument("--sweep-fns", default="exp,log,sqrt,erfc,tanh,sin,cos",
                     help="functors of the precision sweep report")
     args = ap.parse_args()
@@ -486,11 +517,11 @@ def main():
     print(json.dumps(line), flush=True)
     want = set(args.report.split(",")) if args.report else set()
     if "all" in want:
-        want = {"saxpy", "bs_fwd", "sweep", "colfold", "placement"}
+        want = {"saxpy", "bs_fwd", "sweep", "colfold", "placement", "folds"}
     del g, outs, tabs, g64, outs64, tabs64
     torch.cuda.empty_cache()
     for name, fn in (("placement", report_placement), ("saxpy", report_saxpy), ("bs_fwd", report_bs_fwd),
-                     ("colfold", report_colfold), ("sweep", report_sweep)):
+                     ("colfold", report_colfold), ("folds", report_folds), ("sweep", report_sweep)):
         if name in want:
             recs = fn(dev, stream, fns=tuple(args.sweep_fns.split(","))) if name == "sweep" else fn(dev, stream)
             for rec in recs:

@@ -15,10 +15,13 @@
 //    lanes stride over the segment (16-byte loads when aligned), and the
 //    workgroup combines its lanes in a fixed LDS tree;
 //  * column kernel (lo > 1, or short runs): one lane per (output, segment)
-//    folding sequentially; neighbouring lanes read neighbouring addresses;
-//  * combine kernel: when a run is split into segments, one workgroup per
+//    folding sequentially — or, when lo % 4 == 0, per group of 4 adjacent
+//    outputs with one 16-byte load per step, four steps in flight;
+//    neighbouring lanes read neighbouring addresses;
+//  * combine kernels: when a run is split into segments, one workgroup per
 //    output combines the segment summaries (each lane a contiguous range of
-//    segments, then a fixed tree: the same order on every call).
+//    segments, then a fixed tree: the same order on every call); for the
+//    column paths one lane per output combines them in segment order.
 // Arithmetic per functor family:
 //  * sum, asum1, asum2, norm2, mean: FP32 TwoSum-compensated lanes (asum2
 //    adds the exact error of each square), an FP64 combine across segments;
@@ -313,11 +316,58 @@ __global__ __launch_bounds__(kBlock) void fold_col_kernel(Geo g, int64_t segs, i
   L a = F::ident();
   for (int64_t m = m0; m < m1; ++m) F::add(a, g.x[g.at(o, m)], m);
   if (segs > 1) {
-    summ[o * segs + s] = F::summ(a);
+    summ[s * nout + o] = F::summ(a);  // segment-major: coalesced for the combine
     return;
   }
   if constexpr (F::kRescan) out[o] = ext_finish_thread<F>(g, o, a);
   else out[o] = F::finish(F::summ(a), g.mid);
+}
+
+// Column kernel, 16-byte form (lo % 4 == 0, x 16-byte aligned): one lane per
+// (group of 4 adjacent outputs, segment); each step along m loads one f32x4
+// holding element m of the 4 outputs, four steps at a time so that several
+// loads are in flight per lane.
+template <int OP>
+__global__ __launch_bounds__(kBlock) void fold_col4_kernel(Geo g, int64_t segs, int64_t seglen, float* out,
+                                                           Summ* summ) {
+  using F = typename Family<OP>::type;
+  using L = typename F::L;
+  const int64_t nq = g.lo * g.hi / 4;
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= nq * segs) return;
+  const int64_t s = t / nq, q = t - s * nq;
+  const int64_t o = 4 * q;  // first of the 4 outputs; the group never straddles h since lo % 4 == 0
+  const int64_t h = o / g.lo, i = o - h * g.lo;
+  const f32x4* p = reinterpret_cast<const f32x4*>(g.x + i + g.lo * g.mid * h);
+  const int64_t step = g.lo / 4;  // f32x4 elements between m and m + 1
+  const int64_t m0 = s * seglen;
+  const int64_t m1 = (m0 + seglen < g.mid) ? m0 + seglen : g.mid;
+  L a[4] = {F::ident(), F::ident(), F::ident(), F::ident()};
+  int64_t m = m0;
+  for (; m + 4 <= m1; m += 4) {
+    f32x4 v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[u] = __builtin_nontemporal_load(p + (m + u) * step);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) F::add(a[k], v[u][k], m + u);
+    }
+  }
+  for (; m < m1; ++m) {
+    f32x4 v = __builtin_nontemporal_load(p + m * step);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) F::add(a[k], v[k], m);
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (segs > 1) {
+      summ[s * (g.lo * g.hi) + o + k] = F::summ(a[k]);
+    } else {
+      if constexpr (F::kRescan) out[o + k] = ext_finish_thread<F>(g, o + k, a[k]);
+      else out[o + k] = F::finish(F::summ(a[k]), g.mid);
+    }
+  }
 }
 
 // Combine kernel: one workgroup per output; lane l takes a contiguous range
@@ -343,6 +393,22 @@ __global__ __launch_bounds__(kBlock) void fold_combine_kernel(Geo g, int64_t seg
     r = F::finish(a, g.mid);
   }
   if (threadIdx.x == 0) out[o] = r;
+}
+
+// Combine kernel of the column paths: one lane per output folds its segment
+// summaries (segment-major, so neighbouring lanes read neighbouring
+// summaries) in segment order.
+template <int OP>
+__global__ __launch_bounds__(kBlock) void fold_col_combine_kernel(Geo g, int64_t segs, float* out,
+                                                                  const Summ* summ) {
+  using F = typename Family<OP>::type;
+  const int64_t nout = g.lo * g.hi;
+  const int64_t o = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (o >= nout) return;
+  Summ a = summ[o];
+  for (int64_t s = 1; s < segs; ++s) a = F::scomb(a, summ[s * nout + o]);
+  if constexpr (F::kRescan) out[o] = ext_finish_thread<F>(g, o, F::unsumm(a));
+  else out[o] = F::finish(a, g.mid);
 }
 
 // Output for an empty middle extent: NT2 folds nothing from the neutral
@@ -382,7 +448,9 @@ int fold_launch(const float* x, int64_t lo, int64_t mid, int64_t hi, float* out,
     return NT2H_OK;
   }
   // column kernel: aim for >= 2^18 lanes; segments of at least 16 elements
-  int64_t segs = ((int64_t)1 << 18) / nout;
+  const bool col4 = (lo % 4 == 0) && aligned16(x);
+  const int64_t units = col4 ? nout / 4 : nout;  // lanes per segment
+  int64_t segs = ((int64_t)1 << 18) / units;
   int64_t maxsegs = (mid + 15) / 16;
   if (segs > maxsegs) segs = maxsegs;
   if (segs > 4096) segs = 4096;
@@ -394,13 +462,15 @@ int fold_launch(const float* x, int64_t lo, int64_t mid, int64_t hi, float* out,
     if (nout > 0x7fffffffLL) return NT2H_ERR_TOO_LARGE;
     NT2H_TRY(get_scratch(st, (size_t)(nout * segs) * sizeof(Summ), (void**)&summ));
   }
-  const int64_t lanes = nout * segs;
+  const int64_t lanes = units * segs;
   const int64_t blocks = (lanes + kBlock - 1) / kBlock;
   if (blocks > 0x7fffffffLL) return NT2H_ERR_TOO_LARGE;
-  hipLaunchKernelGGL((fold_col_kernel<OP>), dim3((unsigned)blocks), dim3(kBlock), 0, s, g, segs, seglen, out, summ);
+  if (col4) hipLaunchKernelGGL((fold_col4_kernel<OP>), dim3((unsigned)blocks), dim3(kBlock), 0, s, g, segs, seglen, out, summ);
+  else hipLaunchKernelGGL((fold_col_kernel<OP>), dim3((unsigned)blocks), dim3(kBlock), 0, s, g, segs, seglen, out, summ);
   NT2H_HIP(hipGetLastError());
   if (segs > 1) {
-    hipLaunchKernelGGL((fold_combine_kernel<OP>), dim3((unsigned)nout), dim3(kBlock), 0, s, g, segs, out,
+    const int64_t cblocks = (nout + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL((fold_col_combine_kernel<OP>), dim3((unsigned)cblocks), dim3(kBlock), 0, s, g, segs, out,
                        (const Summ*)summ);
     NT2H_HIP(hipGetLastError());
   }

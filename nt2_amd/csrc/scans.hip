@@ -25,10 +25,13 @@
 //    4l..4l+3), scans each tile in the workgroup (wave shuffles, then the
 //    four wave totals through LDS) and carries the tile total forward;
 //  * column kernel (lo > 1, or short runs): one lane per (output, segment)
-//    running the recurrence; neighbouring lanes touch neighbouring addresses;
+//    running the recurrence — or, when lo % 4 == 0, per group of 4 adjacent
+//    outputs with 16-byte loads and stores; neighbouring lanes touch
+//    neighbouring addresses;
 //  * long runs split into segments: pass 1 computes each segment's total,
 //    pass 2 the exclusive prefix of the segment totals per run (one
-//    workgroup per run), pass 3 rescans each segment from its prefix.
+//    workgroup per run; one lane per output on the column paths), pass 3
+//    rescans each segment from its prefix.
 #include <hip/hip_runtime.h>
 
 #include "nt2h_common.hpp"
@@ -256,14 +259,89 @@ __global__ __launch_bounds__(kBlock) void scan_col_kernel(SGeo g, int64_t segs, 
   const int64_t s = t / nout, o = t - s * nout;
   const int64_t m0 = s * seglen;
   const int64_t m1 = (m0 + seglen < g.mid) ? m0 + seglen : g.mid;
-  typename M::C c = (PASS == 3) ? M::unpack(summ[o * segs + s]) : M::cident();
+  typename M::C c = (PASS == 3) ? M::unpack(summ[s * nout + o]) : M::cident();
   typename M::V acc = M::ident();
   for (int64_t m = m0; m < m1; ++m) {
     const int64_t k = g.at(o, m);
     acc = M::comb(acc, M::lift(g.x[k]));
     if (PASS != 1) g.y[k] = M::fin(c, acc);
   }
-  if (PASS == 1) summ[o * segs + s] = M::pack(M::cadd(c, acc));
+  if (PASS == 1) summ[s * nout + o] = M::pack(M::cadd(c, acc));
+}
+
+// Column kernel, 16-byte form (lo % 4 == 0, x and y 16-byte aligned): one
+// lane per (group of 4 adjacent outputs, segment), one f32x4 load and store
+// per step along m, loads issued four steps ahead.
+template <int OP, int PASS>
+__global__ __launch_bounds__(kBlock) void scan_col4_kernel(SGeo g, int64_t segs, int64_t seglen, SSumm* summ) {
+  using M = typename SFam<OP>::type;
+  using V = typename M::V;
+  using C = typename M::C;
+  const int64_t nq = g.lo * g.hi / 4;
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (t >= nq * segs) return;
+  const int64_t s = t / nq, q = t - s * nq;
+  const int64_t o = 4 * q;
+  const int64_t h = o / g.lo, i = o - h * g.lo;
+  const int64_t off = i + g.lo * g.mid * h;
+  const f32x4* px = reinterpret_cast<const f32x4*>(g.x + off);
+  f32x4* py = reinterpret_cast<f32x4*>(g.y + off);
+  const int64_t step = g.lo / 4;
+  const int64_t m0 = s * seglen;
+  const int64_t m1 = (m0 + seglen < g.mid) ? m0 + seglen : g.mid;
+  C c[4];
+  V acc[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    c[k] = (PASS == 3) ? M::unpack(summ[s * (g.lo * g.hi) + o + k]) : M::cident();
+    acc[k] = M::ident();
+  }
+  int64_t m = m0;
+  for (; m + 4 <= m1; m += 4) {
+    f32x4 v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[u] = __builtin_nontemporal_load(px + (m + u) * step);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      f32x4 r;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        acc[k] = M::comb(acc[k], M::lift(v[u][k]));
+        r[k] = M::fin(c[k], acc[k]);
+      }
+      if (PASS != 1) __builtin_nontemporal_store(r, py + (m + u) * step);
+    }
+  }
+  for (; m < m1; ++m) {
+    f32x4 v = __builtin_nontemporal_load(px + m * step);
+    f32x4 r;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      acc[k] = M::comb(acc[k], M::lift(v[k]));
+      r[k] = M::fin(c[k], acc[k]);
+    }
+    if (PASS != 1) __builtin_nontemporal_store(r, py + m * step);
+  }
+  if (PASS == 1) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) summ[s * (g.lo * g.hi) + o + k] = M::pack(M::cadd(c[k], acc[k]));
+  }
+}
+
+// Pass 2 of the column paths: one lane per output replaces its segment
+// totals (segment-major) by their exclusive prefixes, in segment order.
+template <int OP>
+__global__ __launch_bounds__(kBlock) void scan_col_prefix_kernel(int64_t nout, int64_t segs, SSumm* summ) {
+  using M = typename SFam<OP>::type;
+  using C = typename M::C;
+  const int64_t o = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (o >= nout) return;
+  C run = M::cident();
+  for (int64_t s = 0; s < segs; ++s) {
+    C t = M::unpack(summ[s * nout + o]);
+    summ[s * nout + o] = M::pack(run);
+    run = M::ccomb(run, t);
+  }
 }
 
 // Pass 2: per output (one workgroup), replace the segment totals by their
@@ -334,29 +412,38 @@ int scan_launch(const float* x, float* y, int64_t lo, int64_t mid, int64_t hi, n
     NT2H_HIP(hipGetLastError());
     return NT2H_OK;
   }
-  int64_t segs = ((int64_t)1 << 18) / nout;
+  const bool col4 = (lo % 4 == 0) && aligned16(x, y);
+  const int64_t units = col4 ? nout / 4 : nout;  // lanes per segment
+  int64_t segs = ((int64_t)1 << 18) / units;
   int64_t maxsegs = (mid + 63) / 64;
   if (segs > maxsegs) segs = maxsegs;
   if (segs > 4096) segs = 4096;
   if (segs < 1) segs = 1;
   int64_t seglen = (mid + segs - 1) / segs;
   segs = (mid + seglen - 1) / seglen;
-  const int64_t lanes = nout * segs;
+  const int64_t lanes = units * segs;
   const int64_t blocks = (lanes + kBlock - 1) / kBlock;
   if (blocks > 0x7fffffffLL || (segs > 1 && nout > 0x7fffffffLL)) return NT2H_ERR_TOO_LARGE;
   SSumm* summ = nullptr;
+#define NT2H_SCAN_COL(PASS)                                                                                      \
+  do {                                                                                                            \
+    if (col4) hipLaunchKernelGGL((scan_col4_kernel<OP, PASS>), dim3((unsigned)blocks), dim3(kBlock), 0, s, g,   \
+                                 segs, seglen, summ);                                                             \
+    else hipLaunchKernelGGL((scan_col_kernel<OP, PASS>), dim3((unsigned)blocks), dim3(kBlock), 0, s, g, segs,  \
+                            seglen, summ);                                                                        \
+    NT2H_HIP(hipGetLastError());                                                                                  \
+  } while (0)
   if (segs == 1) {
-    hipLaunchKernelGGL((scan_col_kernel<OP, 0>), dim3((unsigned)blocks), dim3(kBlock), 0, s, g, segs, seglen, summ);
-    NT2H_HIP(hipGetLastError());
+    NT2H_SCAN_COL(0);
     return NT2H_OK;
   }
-  NT2H_TRY(get_scratch(st, (size_t)lanes * sizeof(SSumm), (void**)&summ));
-  hipLaunchKernelGGL((scan_col_kernel<OP, 1>), dim3((unsigned)blocks), dim3(kBlock), 0, s, g, segs, seglen, summ);
+  NT2H_TRY(get_scratch(st, (size_t)(nout * segs) * sizeof(SSumm), (void**)&summ));
+  NT2H_SCAN_COL(1);
+  hipLaunchKernelGGL((scan_col_prefix_kernel<OP>), dim3((unsigned)((nout + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
+                     nout, segs, summ);
   NT2H_HIP(hipGetLastError());
-  hipLaunchKernelGGL((scan_prefix_kernel<OP>), dim3((unsigned)nout), dim3(kBlock), 0, s, segs, summ);
-  NT2H_HIP(hipGetLastError());
-  hipLaunchKernelGGL((scan_col_kernel<OP, 3>), dim3((unsigned)blocks), dim3(kBlock), 0, s, g, segs, seglen, summ);
-  NT2H_HIP(hipGetLastError());
+  NT2H_SCAN_COL(3);
+#undef NT2H_SCAN_COL
   return NT2H_OK;
 }
 

@@ -15,9 +15,10 @@ from nt2_amd import ops
 pytestmark = pytest.mark.gpu
 
 # (lo, mid, hi): run kernel single segment, run kernel segmented, short runs
-# (column kernel), outer folds with and without segments, NT2's 3x5x2 table
+# (column kernel), outer folds with and without segments, NT2's 3x5x2 table,
+# and lo % 4 == 0 (the 16-byte column kernels) with and without segments
 SHAPES = [(1, 5000, 37), (1, (1 << 20) + 3, 1), (1, 1 << 18, 2), (1, 100, 1000), (1, 700, 3), (7, 3000, 5),
-          (1000, 3, 7), (3, 5, 2), (2, 40000, 1), (1, 1, 9)]
+          (1000, 3, 7), (3, 5, 2), (2, 40000, 1), (1, 1, 9), (16, 5000, 3), (1024, 300, 2)]
 
 
 def _dev_tensor(a, dev):
