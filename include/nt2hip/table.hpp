@@ -590,6 +590,76 @@ template <class T, class N> inline table<T> sum(const expr<T, N>& e) {
 }
 template <class T> inline table<T> sum(const table<T>& t) { return sum(as_expr(t)); }
 
+// ================================== other reduction_ functors and scans
+// prod, maximum, minimum, asum1, asum2, norm2, mean (core/reduction/include/
+// nt2/core/functions/{prod,...}.hpp) and cumsum, cumprod, cummax, cummin
+// (core/swar) along dimension 1 (1 x cols result), 2 (rows x 1) or a
+// singleton dimension >= 3 (the element itself through the functor), or
+// along the first non-singleton dimension when none is given, as NT2's; the
+// globalX forms fold every element.  Expressions are evaluated into a
+// temporary first; these lower to nt2h_fold_f32 / nt2h_scan_f32 (float).
+namespace detail {
+inline void lmh2(extent x, int dim, int64_t& lo, int64_t& mid, int64_t& hi) {
+  if (dim < 1) throw error("nt2hip: dimensions are 1-based", NT2H_ERR_INVALID_ARG);
+  if (dim == 1) { lo = 1; mid = x.rows; hi = x.cols; }
+  else if (dim == 2) { lo = x.rows; mid = x.cols; hi = 1; }
+  else { lo = x.numel(); mid = 1; hi = 1; }
+}
+inline int first_nonsingleton(extent x) { return x.rows != 1 ? 1 : (x.cols != 1 ? 2 : 1); }
+inline table<float> fold_dim(int op, const table<float>& t, int dim) {
+  const extent x = t.extent_();
+  int64_t lo, mid, hi;
+  lmh2(x, dim, lo, mid, hi);
+  table<float> out(dim == 1 ? 1 : x.rows, dim == 2 ? 1 : x.cols);
+  check(nt2h_fold_f32(op, t.data(), lo, mid, hi, out.data(), abi_stream()), "nt2h_fold_f32");
+  return out;
+}
+inline float fold_all(int op, const table<float>& t) {
+  table<float> out(1, 1);
+  check(nt2h_fold_f32(op, t.data(), 1, t.numel(), 1, out.data(), abi_stream()), "nt2h_fold_f32");
+  return out.to_host()[0];
+}
+inline table<float> scan_dim(int op, const table<float>& t, int dim) {
+  const extent x = t.extent_();
+  int64_t lo, mid, hi;
+  lmh2(x, dim, lo, mid, hi);
+  table<float> out(x.rows, x.cols);
+  check(nt2h_scan_f32(op, t.data(), lo, mid, hi, out.data(), abi_stream()), "nt2h_scan_f32");
+  return out;
+}
+}  // namespace detail
+
+#define NT2HIP_FOLD_FUNCTOR(NAME, GLOBAL, OP)                                                                  \
+  inline table<float> NAME(const table<float>& t, int dim) { return detail::fold_dim(OP, t, dim); }           \
+  inline table<float> NAME(const table<float>& t) {                                                           \
+    return detail::fold_dim(OP, t, detail::first_nonsingleton(t.extent_()));                                   \
+  }                                                                                                            \
+  template <class N> inline table<float> NAME(const expr<float, N>& e, int dim) { return NAME(table<float>(e), dim); } \
+  template <class N> inline table<float> NAME(const expr<float, N>& e) { return NAME(table<float>(e)); }       \
+  inline float GLOBAL(const table<float>& t) { return detail::fold_all(OP, t); }                              \
+  template <class N> inline float GLOBAL(const expr<float, N>& e) { return GLOBAL(table<float>(e)); }
+NT2HIP_FOLD_FUNCTOR(prod, globalprod, NT2H_FOLD_PROD)
+NT2HIP_FOLD_FUNCTOR(maximum, globalmax, NT2H_FOLD_MAXIMUM)
+NT2HIP_FOLD_FUNCTOR(minimum, globalmin, NT2H_FOLD_MINIMUM)
+NT2HIP_FOLD_FUNCTOR(asum1, globalasum1, NT2H_FOLD_ASUM1)
+NT2HIP_FOLD_FUNCTOR(asum2, globalasum2, NT2H_FOLD_ASUM2)
+NT2HIP_FOLD_FUNCTOR(norm2, globalnorm2, NT2H_FOLD_NORM2)
+NT2HIP_FOLD_FUNCTOR(mean, globalmean, NT2H_FOLD_MEAN)
+#undef NT2HIP_FOLD_FUNCTOR
+
+#define NT2HIP_SCAN_FUNCTOR(NAME, OP)                                                                          \
+  inline table<float> NAME(const table<float>& t, int dim) { return detail::scan_dim(OP, t, dim); }           \
+  inline table<float> NAME(const table<float>& t) {                                                           \
+    return detail::scan_dim(OP, t, detail::first_nonsingleton(t.extent_()));                                   \
+  }                                                                                                            \
+  template <class N> inline table<float> NAME(const expr<float, N>& e, int dim) { return NAME(table<float>(e), dim); } \
+  template <class N> inline table<float> NAME(const expr<float, N>& e) { return NAME(table<float>(e)); }
+NT2HIP_SCAN_FUNCTOR(cumsum, NT2H_SCAN_CUMSUM)
+NT2HIP_SCAN_FUNCTOR(cumprod, NT2H_SCAN_CUMPROD)
+NT2HIP_SCAN_FUNCTOR(cummax, NT2H_SCAN_CUMMAX)
+NT2HIP_SCAN_FUNCTOR(cummin, NT2H_SCAN_CUMMIN)
+#undef NT2HIP_SCAN_FUNCTOR
+
 // =============================================================== tie
 template <class T> struct tie2 {
   table<T>& a;

@@ -16,6 +16,8 @@ void nt2ref_saxpy_f32(float a, const float* x, float* y, int64_t n);
 void nt2ref_bs_fwd_f32(const float*, const float*, const float*, const float*, const float*, float*, float*, int64_t);
 void nt2ref_exact_dot_f32(const float* x, const float* y, int64_t n, double* out);
 void nt2ref_exact_sum_f32(const float* x, int64_t n, double* out);
+void nt2ref_fold_lmh_f32(int op, const float* x, int64_t lo, int64_t mid, int64_t hi, float* out);
+void nt2ref_scan_lmh_f32(int op, const float* x, int64_t lo, int64_t mid, int64_t hi, float* y);
 }
 
 using nt2hip::table;
@@ -157,6 +159,66 @@ int main() {
       threw = e.code == NT2H_ERR_INVALID_ARG;
     }
     report("size_mismatch_throws", threw);
+  }
+  {  // maximum / minimum along each dimension: bit-exact vs NT2's fold, NaNs included
+    const int64_t R = 300, C = 7;
+    auto h = uniform(R * C, -4.f, 4.f, 9);
+    h[5] = NAN; h[R + 17] = NAN; h[2 * R + R - 1] = NAN; h[3 * R + 1] = -0.0f;
+    table<float> A(h, R, C);
+    bool ok = true;
+    for (int dim = 1; dim <= 3; ++dim) {
+      int64_t lo = dim == 1 ? 1 : (dim == 2 ? R : R * C), mid = dim == 1 ? R : (dim == 2 ? C : 1),
+              hi = dim == 1 ? C : 1;
+      for (int op : {NT2H_FOLD_MAXIMUM, NT2H_FOLD_MINIMUM}) {
+        std::vector<float> want(lo * hi);
+        nt2ref_fold_lmh_f32(op, h.data(), lo, mid, hi, want.data());
+        table<float> got = op == NT2H_FOLD_MAXIMUM ? nt2hip::maximum(A, dim) : nt2hip::minimum(A, dim);
+        ok = ok && same_bits(got.to_host(), want);
+      }
+    }
+    std::vector<float> want(1);
+    nt2ref_fold_lmh_f32(NT2H_FOLD_MAXIMUM, h.data(), 1, R * C, 1, want.data());
+    ok = ok && bits(nt2hip::globalmax(A)) == bits(want[0]);
+    report("maximum_minimum_dims_bit_exact", ok);
+  }
+  {  // globalmax/globalprod/maximum tables of NT2's unit tests
+    std::vector<float> a{1, 2, 3, 4, 5, 6, 7, 8, 9};
+    table<float> A(a, 3, 3);
+    bool ok = nt2hip::globalmax(A) == 9.f;
+    std::vector<float> b{1, 2, 3, 4};
+    ok = ok && nt2hip::globalprod(table<float>(b, 2, 2)) == 24.f;
+    std::vector<float> y(15);
+    for (int j = 1; j <= 3; ++j)
+      for (int i = 1; i <= 5; ++i) y[(i - 1) + 5 * (j - 1)] = (float)(i + 10 * j);
+    table<float> Y(y, 5, 3);
+    auto m1 = nt2hip::maximum(Y).to_host();        // first non-singleton: dim 1
+    auto m2 = nt2hip::maximum(Y, 2).to_host();
+    ok = ok && m1 == std::vector<float>{15, 25, 35} && m2 == std::vector<float>{31, 32, 33, 34, 35};
+    auto mn = nt2hip::mean(Y, 1).to_host();
+    ok = ok && std::fabs(mn[0] - 13.f) < 1e-5f && std::fabs(mn[2] - 33.f) < 1e-5f;
+    report("fold_unit_tables", ok);
+  }
+  {  // cumsum along 1 and 2 of reshape(1:30, 3, 10): NT2's cumsum test values; cummax bit-exact
+    std::vector<float> a(30);
+    for (int i = 0; i < 30; ++i) a[i] = (float)(i + 1);
+    table<float> A(a, 3, 10);
+    std::vector<float> w1(30), w2(30);
+    nt2ref_scan_lmh_f32(NT2H_SCAN_CUMSUM, a.data(), 1, 3, 10, w1.data());
+    nt2ref_scan_lmh_f32(NT2H_SCAN_CUMSUM, a.data(), 3, 10, 1, w2.data());
+    bool ok = nt2hip::cumsum(A).to_host() == w1 && nt2hip::cumsum(A, 2).to_host() == w2 && w1[8] == 24.f;
+    auto h = uniform(5000, -1.f, 1.f, 11);
+    h[100] = NAN;
+    table<float> H(h, 5000, 1);
+    std::vector<float> wm(5000);
+    nt2ref_scan_lmh_f32(NT2H_SCAN_CUMMAX, h.data(), 1, 5000, 1, wm.data());
+    ok = ok && same_bits(nt2hip::cummax(H).to_host(), wm);
+    // an expression operand is evaluated first
+    table<float> E = nt2hip::cummin(-H);
+    std::vector<float> nh(5000), wn(5000);
+    for (int i = 0; i < 5000; ++i) nh[i] = -h[i];
+    nt2ref_scan_lmh_f32(NT2H_SCAN_CUMMIN, nh.data(), 1, 5000, 1, wn.data());
+    ok = ok && same_bits(E.to_host(), wn);
+    report("scans_unit_and_bit_exact", ok);
   }
   std::printf("%d failure(s)\n", failures);
   return failures ? 1 : 0;
