@@ -302,6 +302,51 @@ def report_folds(dev, stream):
     return lines
 
 
+def report_host(dev, stream, n=1 << 24):
+    """Host tables: staged pageable copies (nt2h_upload / nt2h_download) against
+    hipMemcpy from pageable and from pinned memory, and the Black-Scholes
+    fwd+adj over host tables (PCIe-inclusive options/s, blocks of 2^20
+    options overlapped on two streams)."""
+    import torch
+
+    from nt2_amd import call, synth
+
+    st = torch.cuda.current_stream().cuda_stream
+    nbytes = 256 << 20
+    host = np.ones(nbytes, np.uint8)
+    back = np.zeros(nbytes, np.uint8)
+    d = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+    pinned = torch.ones(nbytes, dtype=torch.uint8).pin_memory()
+    th = torch.from_numpy(host)
+
+    def wall(fn, reps=4):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps
+
+    rec = {"report": "host", "bytes": nbytes}
+    rec["upload_staged_gbps"] = nbytes / wall(lambda: call("nt2h_upload", d.data_ptr(), host.ctypes.data, nbytes, st)) / 1e9
+    rec["upload_hipmemcpy_pageable_gbps"] = nbytes / wall(lambda: d.copy_(th)) / 1e9
+    rec["upload_pinned_gbps"] = nbytes / wall(lambda: d.copy_(pinned, non_blocking=True)) / 1e9
+    rec["download_staged_gbps"] = nbytes / wall(lambda: call("nt2h_download", back.ctypes.data, d.data_ptr(), nbytes, st)) / 1e9
+    rec["download_hipmemcpy_pageable_gbps"] = nbytes / wall(lambda: th.copy_(d)) / 1e9
+    rec["download_pinned_gbps"] = nbytes / wall(lambda: pinned.copy_(d, non_blocking=True)) / 1e9
+    del d, pinned
+    dd = synth.bs_inputs(n, with_w=True)
+    ins = [dd[k] for k in ("S", "K", "T", "r", "sig", "w")]
+    outs = [np.empty(n, np.float32) for _ in range(6)]
+    tot = np.zeros(2)
+    args = [a.ctypes.data for a in ins] + [o.ctypes.data for o in outs] + [tot.ctypes.data, n, 1 << 20]
+    t = wall(lambda: call("nt2h_bs_fwd_adj_f32_host", *args), 3)
+    rec["bs_fwd_adj_host"] = {"options": n, "block": 1 << 20, "s": t, "options_per_s": n / t,
+                              "pcie_gbps": 48 * n / t / 1e9}
+    return [rec]
+
+
 def report_colfold(dev, stream, rows=16384, cols=8192):
     """Column fold + adjoint over 16384 x 8192: time, and error vs a torch FP64 evaluation on 64 columns."""
     import torch
@@ -335,7 +380,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--report", default="", help="extra report lines after the headline: "
-                    "comma list of placement,sweep,bs_fwd,saxpy,colfold,folds (or all)")
+                    "comma list of placement,sweep,bs_fwd,saxpy,colfold,folds,host (or all)")
     ap.add_argument("--sweep-fns", default="exp,log,sqrt,erfc,tanh,sin,cos",
                     help="functors of the precision sweep report")
     args = ap.parse_args()
@@ -517,11 +562,12 @@ def main():
     print(json.dumps(line), flush=True)
     want = set(args.report.split(",")) if args.report else set()
     if "all" in want:
-        want = {"saxpy", "bs_fwd", "sweep", "colfold", "placement", "folds"}
+        want = {"saxpy", "bs_fwd", "sweep", "colfold", "placement", "folds", "host"}
     del g, outs, tabs, g64, outs64, tabs64
     torch.cuda.empty_cache()
     for name, fn in (("placement", report_placement), ("saxpy", report_saxpy), ("bs_fwd", report_bs_fwd),
-                     ("colfold", report_colfold), ("folds", report_folds), ("sweep", report_sweep)):
+                     ("colfold", report_colfold), ("folds", report_folds), ("host", report_host),
+                     ("sweep", report_sweep)):
         if name in want:
             recs = fn(dev, stream, fns=tuple(args.sweep_fns.split(","))) if name == "sweep" else fn(dev, stream)
             for rec in recs:

@@ -94,6 +94,13 @@ int nt2h_free(void* ptr);
 int nt2h_host_alloc(void** ptr, size_t bytes); /* pinned host memory */
 int nt2h_host_free(void* ptr);
 int nt2h_memcpy(void* dst, const void* src, size_t bytes, int kind, nt2h_stream_t s);
+/* Pageable host memory <-> device, staged through pinned buffers in 8 MiB
+ * chunks with the host-side copy of one chunk overlapping the DMA of the
+ * next (NT2's blocked host transfers: sdk/cuda/settings/specific_cuda.hpp
+ * cu_buffers, transfer_htd/transfer_dth).  Synchronous: on return dst holds
+ * the data. */
+int nt2h_upload(void* dst_device, const void* src_host, size_t bytes, nt2h_stream_t s);
+int nt2h_download(void* dst_host, const void* src_device, size_t bytes, nt2h_stream_t s);
 int nt2h_sync(nt2h_stream_t s);
 
 /* ---------------------------------------------------- elementwise (transform_)
@@ -194,6 +201,14 @@ int nt2h_scan_f32(int op, const float* x, int64_t lo, int64_t mid, int64_t hi, f
 int nt2h_bs_fwd_adj_f32(const float* S, const float* K, const float* T, const float* r, const float* sig,
                         const float* w, float* price, float* dS, float* dK, float* dT, float* dr, float* dsig,
                         double* totals, int64_t n, nt2h_stream_t s);
+/* The same over host tables (pageable memory): blocks of `block` options
+ * (<= 0: 2^20) alternate between two streams so that staging, H2D, the
+ * kernel and D2H of neighbouring blocks overlap; totals (host memory, may be
+ * NULL) are the FP64 sum of the block totals in block order.  Outputs equal
+ * nt2h_bs_fwd_adj_f32's bit for bit.  Synchronous. */
+int nt2h_bs_fwd_adj_f32_host(const float* S, const float* K, const float* T, const float* r, const float* sig,
+                             const float* w, float* price, float* dS, float* dK, float* dT, float* dr, float* dsig,
+                             double* totals, int64_t n, int64_t block);
 int nt2h_bs_fwd_adj_f64(const double* S, const double* K, const double* T, const double* r,
                         const double* sig, const double* w, double* price, double* dS, double* dK, double* dT,
                         double* dr, double* dsig, double* totals, int64_t n, nt2h_stream_t s);

@@ -420,15 +420,15 @@ template <class T> class table {
     }
     ext_ = e;
   }
+  // host <-> device through the library's pinned staging (synchronous)
   void upload(const std::vector<T>& host) {
     if ((int64_t)host.size() != numel()) throw error("nt2hip::table upload size", NT2H_ERR_INVALID_ARG);
-    check(nt2h_memcpy(p_, host.data(), sizeof(T) * host.size(), NT2H_H2D, abi_stream()), "nt2h_memcpy");
-    check(nt2h_sync(abi_stream()), "nt2h_sync");
+    check(nt2h_upload(p_, host.data(), sizeof(T) * host.size(), abi_stream()), "nt2h_upload");
   }
   std::vector<T> to_host() const {
     std::vector<T> h(numel());
-    check(nt2h_memcpy(h.data(), p_, sizeof(T) * h.size(), NT2H_D2H, abi_stream()), "nt2h_memcpy");
-    check(nt2h_sync(abi_stream()), "nt2h_sync");
+    check(nt2h_sync(abi_stream()), "nt2h_sync");  // work queued on the table's stream first
+    check(nt2h_download(h.data(), p_, sizeof(T) * h.size(), abi_stream()), "nt2h_download");
     return h;
   }
 

@@ -101,6 +101,17 @@ def test_fold_scan_argument_checks():
     assert L.nt2h_scan_f32(0, None, 1, 0, 1, None, None) == 0  # empty
 
 
+def test_transfer_argument_checks():
+    L = nt2_amd.lib()
+    f = fake()
+    assert L.nt2h_upload(None, f, 16, None) == -2
+    assert L.nt2h_download(f, None, 16, None) == -2
+    assert L.nt2h_upload(None, None, 0, None) == 0
+    a12 = [f] * 12
+    assert L.nt2h_bs_fwd_adj_f32_host(*a12[:11], None, None, 8, 0) == -2
+    assert L.nt2h_bs_fwd_adj_f32_host(*a12, None, -8, 0) == -1
+
+
 def test_runtime_argument_checks():
     L = nt2_amd.lib()
     assert L.nt2h_malloc(None, 16) == -2
