@@ -149,12 +149,13 @@ int nt2h_dot_f64(const double* x, const double* y, int64_t n, double* out, nt2h_
  * norm2 = sqrt(asum2), mean = rec(size) * sum (core/reduction/include/nt2/
  * core/functions/expr/{asum1,asum2,norm2,mean}.hpp; boost/simd/reduction/
  * functions/{sum,prod,maximum,minimum}.hpp).  Whole-table globalX(x) is
- * lo = hi = 1, mid = numel.  out holds lo*hi floats.
+ * lo = hi = 1, mid = numel.  out holds lo*hi values of the table type.
  *  - maximum/minimum are bit-identical to NT2's left-to-right a > b ? a : b
- *    fold from Maxinit = -FLT_MAX (Valmax = FLT_MAX), NaN and signed-zero
- *    behaviour included.
- *  - sum-like folds are FP32-compensated with an FP64 cross-segment combine
- *    (more accurate than NT2's sequential float order, so not bit-identical);
+ *    fold from Maxinit = -FLT_MAX (Valmax = FLT_MAX; -DBL_MAX/DBL_MAX for
+ *    double), NaN and signed-zero behaviour included.
+ *  - sum-like folds are compensated in the table's type with a compensated
+ *    FP64 cross-segment combine (more accurate than NT2's sequential order,
+ *    so not bit-identical);
  *    prod keeps an FP64 mantissa with a separate exponent, so partial
  *    products never overflow (NT2's float product can, giving inf).
  *  - mid == 0 yields the neutral element (mean: NaN, as rec(0)*0). */
@@ -169,6 +170,7 @@ typedef enum nt2h_fold_op {
   NT2H_FOLD_MEAN = 7
 } nt2h_fold_op;
 int nt2h_fold_f32(int op, const float* x, int64_t lo, int64_t mid, int64_t hi, float* out, nt2h_stream_t s);
+int nt2h_fold_f64(int op, const double* x, int64_t lo, int64_t mid, int64_t hi, double* out, nt2h_stream_t s);
 
 /* Scans along one dimension (NT2 inner_scan / outer_scan over the middle
  * index of the same (lo, mid, hi) reshaping; core/swar/include/nt2/core/
@@ -187,6 +189,7 @@ typedef enum nt2h_scan_op {
   NT2H_SCAN_CUMMIN = 3
 } nt2h_scan_op;
 int nt2h_scan_f32(int op, const float* x, int64_t lo, int64_t mid, int64_t hi, float* y, nt2h_stream_t s);
+int nt2h_scan_f64(int op, const double* x, int64_t lo, int64_t mid, int64_t hi, double* y, nt2h_stream_t s);
 
 /* ------------------------------------------------------------- adjoints
  * NT2 has no differentiation; these pair a forward loop nest with its

@@ -10,6 +10,10 @@ namespace nt2h {
 // Native clang vectors: 128-bit loads/stores that the nontemporal builtins accept.
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x2 __attribute__((ext_vector_type(2)));
+// The 16-byte vector of an element type and its width.
+template <class T> struct Vec16;
+template <> struct Vec16<float> { using type = f32x4; static constexpr int n = 4; };
+template <> struct Vec16<double> { using type = f64x2; static constexpr int n = 2; };
 
 constexpr int kBlock = 256;          // 4 wave64 per workgroup
 constexpr int kWaves = kBlock / 64;

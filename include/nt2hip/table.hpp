@@ -597,7 +597,7 @@ template <class T> inline table<T> sum(const table<T>& t) { return sum(as_expr(t
 // singleton dimension >= 3 (the element itself through the functor), or
 // along the first non-singleton dimension when none is given, as NT2's; the
 // globalX forms fold every element.  Expressions are evaluated into a
-// temporary first; these lower to nt2h_fold_f32 / nt2h_scan_f32 (float).
+// temporary first; these lower to nt2h_fold_f32/f64 and nt2h_scan_f32/f64.
 namespace detail {
 inline void lmh2(extent x, int dim, int64_t& lo, int64_t& mid, int64_t& hi) {
   if (dim < 1) throw error("nt2hip: dimensions are 1-based", NT2H_ERR_INVALID_ARG);
@@ -606,38 +606,50 @@ inline void lmh2(extent x, int dim, int64_t& lo, int64_t& mid, int64_t& hi) {
   else { lo = x.numel(); mid = 1; hi = 1; }
 }
 inline int first_nonsingleton(extent x) { return x.rows != 1 ? 1 : (x.cols != 1 ? 2 : 1); }
-inline table<float> fold_dim(int op, const table<float>& t, int dim) {
+inline int fold_abi(int op, const float* x, int64_t lo, int64_t mid, int64_t hi, float* out) {
+  return nt2h_fold_f32(op, x, lo, mid, hi, out, abi_stream());
+}
+inline int fold_abi(int op, const double* x, int64_t lo, int64_t mid, int64_t hi, double* out) {
+  return nt2h_fold_f64(op, x, lo, mid, hi, out, abi_stream());
+}
+inline int scan_abi(int op, const float* x, int64_t lo, int64_t mid, int64_t hi, float* y) {
+  return nt2h_scan_f32(op, x, lo, mid, hi, y, abi_stream());
+}
+inline int scan_abi(int op, const double* x, int64_t lo, int64_t mid, int64_t hi, double* y) {
+  return nt2h_scan_f64(op, x, lo, mid, hi, y, abi_stream());
+}
+template <class T> inline table<T> fold_dim(int op, const table<T>& t, int dim) {
   const extent x = t.extent_();
   int64_t lo, mid, hi;
   lmh2(x, dim, lo, mid, hi);
-  table<float> out(dim == 1 ? 1 : x.rows, dim == 2 ? 1 : x.cols);
-  check(nt2h_fold_f32(op, t.data(), lo, mid, hi, out.data(), abi_stream()), "nt2h_fold_f32");
+  table<T> out(dim == 1 ? 1 : x.rows, dim == 2 ? 1 : x.cols);
+  check(fold_abi(op, t.data(), lo, mid, hi, out.data()), "nt2h_fold");
   return out;
 }
-inline float fold_all(int op, const table<float>& t) {
-  table<float> out(1, 1);
-  check(nt2h_fold_f32(op, t.data(), 1, t.numel(), 1, out.data(), abi_stream()), "nt2h_fold_f32");
+template <class T> inline T fold_all(int op, const table<T>& t) {
+  table<T> out(1, 1);
+  check(fold_abi(op, t.data(), 1, t.numel(), 1, out.data()), "nt2h_fold");
   return out.to_host()[0];
 }
-inline table<float> scan_dim(int op, const table<float>& t, int dim) {
+template <class T> inline table<T> scan_dim(int op, const table<T>& t, int dim) {
   const extent x = t.extent_();
   int64_t lo, mid, hi;
   lmh2(x, dim, lo, mid, hi);
-  table<float> out(x.rows, x.cols);
-  check(nt2h_scan_f32(op, t.data(), lo, mid, hi, out.data(), abi_stream()), "nt2h_scan_f32");
+  table<T> out(x.rows, x.cols);
+  check(scan_abi(op, t.data(), lo, mid, hi, out.data()), "nt2h_scan");
   return out;
 }
 }  // namespace detail
 
 #define NT2HIP_FOLD_FUNCTOR(NAME, GLOBAL, OP)                                                                  \
-  inline table<float> NAME(const table<float>& t, int dim) { return detail::fold_dim(OP, t, dim); }           \
-  inline table<float> NAME(const table<float>& t) {                                                           \
+  template <class T> inline table<T> NAME(const table<T>& t, int dim) { return detail::fold_dim(OP, t, dim); } \
+  template <class T> inline table<T> NAME(const table<T>& t) {                                                 \
     return detail::fold_dim(OP, t, detail::first_nonsingleton(t.extent_()));                                   \
   }                                                                                                            \
-  template <class N> inline table<float> NAME(const expr<float, N>& e, int dim) { return NAME(table<float>(e), dim); } \
-  template <class N> inline table<float> NAME(const expr<float, N>& e) { return NAME(table<float>(e)); }       \
-  inline float GLOBAL(const table<float>& t) { return detail::fold_all(OP, t); }                              \
-  template <class N> inline float GLOBAL(const expr<float, N>& e) { return GLOBAL(table<float>(e)); }
+  template <class T, class N> inline table<T> NAME(const expr<T, N>& e, int dim) { return NAME(table<T>(e), dim); } \
+  template <class T, class N> inline table<T> NAME(const expr<T, N>& e) { return NAME(table<T>(e)); }         \
+  template <class T> inline T GLOBAL(const table<T>& t) { return detail::fold_all(OP, t); }                    \
+  template <class T, class N> inline T GLOBAL(const expr<T, N>& e) { return GLOBAL(table<T>(e)); }
 NT2HIP_FOLD_FUNCTOR(prod, globalprod, NT2H_FOLD_PROD)
 NT2HIP_FOLD_FUNCTOR(maximum, globalmax, NT2H_FOLD_MAXIMUM)
 NT2HIP_FOLD_FUNCTOR(minimum, globalmin, NT2H_FOLD_MINIMUM)
@@ -648,12 +660,12 @@ NT2HIP_FOLD_FUNCTOR(mean, globalmean, NT2H_FOLD_MEAN)
 #undef NT2HIP_FOLD_FUNCTOR
 
 #define NT2HIP_SCAN_FUNCTOR(NAME, OP)                                                                          \
-  inline table<float> NAME(const table<float>& t, int dim) { return detail::scan_dim(OP, t, dim); }           \
-  inline table<float> NAME(const table<float>& t) {                                                           \
+  template <class T> inline table<T> NAME(const table<T>& t, int dim) { return detail::scan_dim(OP, t, dim); } \
+  template <class T> inline table<T> NAME(const table<T>& t) {                                                 \
     return detail::scan_dim(OP, t, detail::first_nonsingleton(t.extent_()));                                   \
   }                                                                                                            \
-  template <class N> inline table<float> NAME(const expr<float, N>& e, int dim) { return NAME(table<float>(e), dim); } \
-  template <class N> inline table<float> NAME(const expr<float, N>& e) { return NAME(table<float>(e)); }
+  template <class T, class N> inline table<T> NAME(const expr<T, N>& e, int dim) { return NAME(table<T>(e), dim); } \
+  template <class T, class N> inline table<T> NAME(const expr<T, N>& e) { return NAME(table<T>(e)); }
 NT2HIP_SCAN_FUNCTOR(cumsum, NT2H_SCAN_CUMSUM)
 NT2HIP_SCAN_FUNCTOR(cumprod, NT2H_SCAN_CUMPROD)
 NT2HIP_SCAN_FUNCTOR(cummax, NT2H_SCAN_CUMMAX)

@@ -60,6 +60,8 @@ SIGNATURES = {
     "nt2h_bs_fwd_adj_f64": (_I, [_P] * 13 + [_I64, _P]),
     "nt2h_colfold_gc_f32": (_I, [_P, _I64, _I64, _I64] + [_P] * 8 + [_P]),
     "nt2h_fold_f32": (_I, [_I, _P, _I64, _I64, _I64, _P, _P]),
+    "nt2h_fold_f64": (_I, [_I, _P, _I64, _I64, _I64, _P, _P]),
+    "nt2h_scan_f64": (_I, [_I, _P, _I64, _I64, _I64, _P, _P]),
     "nt2h_upload": (_I, [_P, _P, _SZ, _P]),
     "nt2h_download": (_I, [_P, _P, _SZ, _P]),
     "nt2h_bs_fwd_adj_f32_host": (_I, [_P] * 13 + [_I64, _I64]),

@@ -15,10 +15,6 @@
 
 namespace nt2h {
 
-template <class T> struct Vec16;
-template <> struct Vec16<float> { using type = f32x4; static constexpr int n = 4; };
-template <> struct Vec16<double> { using type = f64x2; static constexpr int n = 2; };
-
 template <class V, class T> __device__ __forceinline__ T vget(const V& v, int i) {
   return reinterpret_cast<const T*>(&v)[i];
 }

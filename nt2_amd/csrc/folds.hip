@@ -1,6 +1,6 @@
 // The other reduction_ functors of NT2 (prod, maximum, minimum, asum1,
-// asum2, norm2, mean, and sum with a float result) over one dimension of a
-// contiguous table.
+// asum2, norm2, mean, and sum with a table-typed result) over one dimension
+// of a contiguous float or double table.
 //
 // NT2 reshapes a reduction along dimension k into (lo, mid, hi) = (product
 // of the dimensions before k, dimension k, product of those after) and runs
@@ -15,21 +15,23 @@
 //    lanes stride over the segment (16-byte loads when aligned), and the
 //    workgroup combines its lanes in a fixed LDS tree;
 //  * column kernel (lo > 1, or short runs): one lane per (output, segment)
-//    folding sequentially — or, when lo % 4 == 0, per group of 4 adjacent
-//    outputs with one 16-byte load per step, four steps in flight;
+//    folding sequentially — or, when lo is a multiple of the 16-byte width
+//    W (4 floats, 2 doubles), per group of W adjacent outputs with one
+//    16-byte load per step, four steps in flight;
 //    neighbouring lanes read neighbouring addresses;
 //  * combine kernels: when a run is split into segments, one workgroup per
 //    output combines the segment summaries (each lane a contiguous range of
 //    segments, then a fixed tree: the same order on every call); for the
 //    column paths one lane per output combines them in segment order.
 // Arithmetic per functor family:
-//  * sum, asum1, asum2, norm2, mean: FP32 TwoSum-compensated lanes (asum2
-//    adds the exact error of each square), an FP64 combine across segments;
-//    the float total then takes NT2's last steps: sqrt for norm2,
-//    rec(float(mid)) * total for mean (expr/mean.hpp:22-23);
-//  * prod: an FP64 mantissa with a separate integer exponent, renormalised
-//    every four factors, so that no partial product overflows or
-//    underflows; the result is rounded to float once;
+//  * sum, asum1, asum2, norm2, mean: TwoSum-compensated lanes in the table's
+//    type (asum2 adds the exact error of each square), compensated FP64
+//    pairs across segments; the total then takes NT2's last steps: sqrt for
+//    norm2, rec(T(mid)) * total for mean (expr/mean.hpp:22-23);
+//  * prod: an FP64 mantissa with a separate integer exponent (double
+//    factors enter as their frexp mantissa), renormalised every four
+//    factors, so that no partial product overflows or underflows; the
+//    result is rounded to the table's type once;
 //  * maximum, minimum: NT2's selection a > b ? a : b (a < b ? a : b) folded
 //    from Maxinit = -FLT_MAX (Valmax = FLT_MAX) returns, for a fold with no
 //    NaN, the latest element equal to the extreme value, or the neutral if
@@ -57,8 +59,8 @@ enum FoldOpId {
 };
 
 // Table geometry: element m of output o = i + lo*h sits at i + lo*(m + mid*h).
-struct Geo {
-  const float* x;
+template <class T> struct Geo {
+  const T* x;
   int64_t lo, mid, hi;
   __device__ __forceinline__ int64_t at(int64_t o, int64_t m) const {
     if (lo == 1) return o * mid + m;
@@ -67,24 +69,53 @@ struct Geo {
   }
 };
 
-// Segment summaries live in scratch as 4 doubles each, index o*segs + seg.
+// Segment summaries live in scratch as 4 doubles each.
 struct Summ {
   double w[4];
 };
 
+__device__ __forceinline__ float fabs_t(float x) { return __builtin_fabsf(x); }
+__device__ __forceinline__ double fabs_t(double x) { return __builtin_fabs(x); }
+__device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float sqrt_t(float x) { return __builtin_sqrtf(x); }
+__device__ __forceinline__ double sqrt_t(double x) { return __builtin_sqrt(x); }
+
+// Error-free sum in either precision.
+template <class T> __device__ __forceinline__ void two_sum_t(T a, T b, T& s, T& e) {
+  s = a + b;
+  T bp = s - a;
+  e = (a - (s - bp)) + (b - bp);
+}
+template <class T> struct CompT {
+  T s, c;
+  __device__ __forceinline__ void add(T x) {
+    T t, e;
+    two_sum_t(s, x, t, e);
+    s = t;
+    c += e;
+  }
+  __device__ __forceinline__ void merge(T s2, T c2) {
+    T t, e;
+    two_sum_t(s, s2, t, e);
+    s = t;
+    c = c + c2 + e;
+  }
+};
+
 // ------------------------------------------------------------ sum family
-template <int OP> struct FSum {
+template <int OP, class T> struct FSum {
   struct L {
-    Comp c;
+    CompT<T> c;
   };
-  __device__ static L ident() { return L{{0.0f, 0.0f}}; }
-  __device__ static void add(L& a, float x, int64_t) {
+  __device__ static L ident() { return L{{T(0), T(0)}}; }
+  __device__ static void add(L& a, T x, int64_t) {
     if (OP == F_ASUM1) {
-      a.c.add(__builtin_fabsf(x));
+      a.c.add(fabs_t(x));
     } else if (OP == F_ASUM2 || OP == F_NORM2) {
-      float ax = __builtin_fabsf(x);
-      float p = ax * ax;
-      float e = __builtin_fmaf(ax, ax, -p);  // exact error of the square
+      T ax = fabs_t(x);
+      T p = ax * ax;
+      T e = fma_t(ax, ax, -p);  // exact error of the square
       a.c.add(p);
       a.c.c += e;
     } else {
@@ -95,17 +126,25 @@ template <int OP> struct FSum {
     a.c.merge(b.c.s, b.c.c);
     return a;
   }
-  __device__ static Summ summ(const L& a) { return Summ{{(double)a.c.s + (double)a.c.c, 0.0, 0.0, 0.0}}; }
+  // summaries: a compensated FP64 pair (s, c)
+  __device__ static Summ summ(const L& a) {
+    double s, e;
+    two_sum_t((double)a.c.s, (double)a.c.c, s, e);
+    return Summ{{s, e, 0.0, 0.0}};
+  }
   __device__ static Summ scomb(Summ a, const Summ& b) {
-    a.w[0] += b.w[0];
+    double t, e;
+    two_sum_t(a.w[0], b.w[0], t, e);
+    a.w[1] = a.w[1] + b.w[1] + e;
+    a.w[0] = t;
     return a;
   }
   __device__ static Summ sident() { return Summ{{0.0, 0.0, 0.0, 0.0}}; }
-  __device__ static float finish(const Summ& a, int64_t mid) {
-    float t = (float)a.w[0];
-    if (OP == F_NORM2) return __builtin_sqrtf(t);
+  __device__ static T finish(const Summ& a, int64_t mid) {
+    T t = (T)(a.w[0] + a.w[1]);
+    if (OP == F_NORM2) return sqrt_t(t);
     if (OP == F_MEAN) {
-      float f = 1.0f / (float)mid;  // NT2: rec(value_type(size)) * sum
+      T f = T(1) / (T)mid;  // NT2: rec(value_type(size)) * sum
       return f * t;
     }
     return t;
@@ -114,7 +153,7 @@ template <int OP> struct FSum {
 };
 
 // ----------------------------------------------------------- prod family
-struct FProd {
+template <class T> struct FProd {
   struct L {
     double m;
     int e;
@@ -125,15 +164,25 @@ struct FProd {
       int k;
       a.m = __builtin_frexp(a.m, &k);
       a.e += k;
-      // keep the exponent bounded: beyond these the float result is 0 or inf
+      // keep the exponent bounded: beyond these the result is 0 or inf in either type
       a.e = a.e > (1 << 28) ? (1 << 28) : (a.e < -(1 << 28) ? -(1 << 28) : a.e);
     }
     a.k = 0;
   }
   __device__ static L ident() { return L{1.0, 0, 0}; }
-  __device__ static void add(L& a, float x, int64_t) {
-    a.m *= (double)x;
-    if (++a.k == 4) norm(a);  // |x| < 2^128: four factors stay inside double's range
+  __device__ static void add(L& a, T x, int64_t) {
+    if (sizeof(T) == 8) {  // a double factor enters as its mantissa in [0.5, 1)
+      double xm = (double)x;
+      if (xm != 0.0 && __builtin_isfinite(xm)) {
+        int k;
+        xm = __builtin_frexp(xm, &k);
+        a.e += k;
+      }
+      a.m *= xm;
+    } else {
+      a.m *= (double)x;  // |x| < 2^128: four factors stay inside double's range
+    }
+    if (++a.k == 4) norm(a);
   }
   __device__ static L comb(L a, L b) {
     norm(a);
@@ -153,29 +202,29 @@ struct FProd {
     return Summ{{r.m, (double)r.e, 0.0, 0.0}};
   }
   __device__ static Summ sident() { return Summ{{1.0, 0.0, 0.0, 0.0}}; }
-  __device__ static float finish(const Summ& a, int64_t) {
+  __device__ static T finish(const Summ& a, int64_t) {
     double m = a.w[0];
     int e = (int)a.w[1];
-    if (m == 0.0 || !__builtin_isfinite(m)) return (float)m;
+    if (m == 0.0 || !__builtin_isfinite(m)) return (T)m;
     // |m| in [0.5, 1): clamp e so that ldexp is exact or saturates the same way
     if (e > 2000) e = 2000;
     if (e < -2000) e = -2000;
-    return (float)__builtin_ldexp(m, e);
+    return (T)__builtin_ldexp(m, e);
   }
   static constexpr bool kRescan = false;
 };
 
 // -------------------------------------------------- maximum / minimum
-template <bool MAX> struct FExt {
+template <bool MAX, class T> struct FExt {
   struct L {
     int64_t nan;  // index of the last NaN, -1 if none
     int64_t j;    // index of the latest extreme, -1 if none
-    float v;
+    T v;
   };
-  __device__ static bool better(float a, float b) { return MAX ? (a > b) : (a < b); }
-  __device__ static L ident() { return L{-1, -1, 0.0f}; }
+  __device__ static bool better(T a, T b) { return MAX ? (a > b) : (a < b); }
+  __device__ static L ident() { return L{-1, -1, T(0)}; }
   // elements arrive in increasing index order within a lane
-  __device__ static void add(L& a, float x, int64_t idx) {
+  __device__ static void add(L& a, T x, int64_t idx) {
     if (x != x) {
       a.nan = idx;
     } else if (a.j < 0 || !better(a.v, x)) {
@@ -197,31 +246,35 @@ template <bool MAX> struct FExt {
     return r;
   }
   __device__ static Summ summ(const L& a) { return Summ{{(double)a.nan, (double)a.j, (double)a.v, 0.0}}; }
-  __device__ static L unsumm(const Summ& s) { return L{(int64_t)s.w[0], (int64_t)s.w[1], (float)s.w[2]}; }
+  __device__ static L unsumm(const Summ& s) { return L{(int64_t)s.w[0], (int64_t)s.w[1], (T)s.w[2]}; }
   __device__ static Summ scomb(const Summ& a, const Summ& b) { return summ(comb(unsumm(a), unsumm(b))); }
   __device__ static Summ sident() { return summ(ident()); }
-  static constexpr float neutral() { return MAX ? -0x1.fffffep+127f : 0x1.fffffep+127f; }
+  // Maxinit / Valmax: the largest finite magnitude of T
+  __device__ static T neutral() {
+    if (sizeof(T) == 4) return (T)(MAX ? -0x1.fffffep+127f : 0x1.fffffep+127f);
+    return (T)(MAX ? -0x1.fffffffffffffp+1023 : 0x1.fffffffffffffp+1023);
+  }
   // no NaN: the latest extreme unless the neutral is more extreme
-  __device__ static float pick(const L& a) {
+  __device__ static T pick(const L& a) {
     if (a.j < 0) return neutral();
     return better(neutral(), a.v) ? neutral() : a.v;
   }
   static constexpr bool kRescan = true;
 };
 
-template <int OP> struct Family;
-template <> struct Family<F_SUM> { using type = FSum<F_SUM>; };
-template <> struct Family<F_ASUM1> { using type = FSum<F_ASUM1>; };
-template <> struct Family<F_ASUM2> { using type = FSum<F_ASUM2>; };
-template <> struct Family<F_NORM2> { using type = FSum<F_NORM2>; };
-template <> struct Family<F_MEAN> { using type = FSum<F_MEAN>; };
-template <> struct Family<F_PROD> { using type = FProd; };
-template <> struct Family<F_MAX> { using type = FExt<true>; };
-template <> struct Family<F_MIN> { using type = FExt<false>; };
+template <int OP, class T> struct Family;
+template <class T> struct Family<F_SUM, T> { using type = FSum<F_SUM, T>; };
+template <class T> struct Family<F_ASUM1, T> { using type = FSum<F_ASUM1, T>; };
+template <class T> struct Family<F_ASUM2, T> { using type = FSum<F_ASUM2, T>; };
+template <class T> struct Family<F_NORM2, T> { using type = FSum<F_NORM2, T>; };
+template <class T> struct Family<F_MEAN, T> { using type = FSum<F_MEAN, T>; };
+template <class T> struct Family<F_PROD, T> { using type = FProd<T>; };
+template <class T> struct Family<F_MAX, T> { using type = FExt<true, T>; };
+template <class T> struct Family<F_MIN, T> { using type = FExt<false, T>; };
 
 // Fixed-order LDS tree over the workgroup; every thread receives the result.
-template <class T, class Comb>
-__device__ __forceinline__ T block_tree(T v, T* sh, Comb comb) {
+template <class S, class Comb>
+__device__ __forceinline__ S block_tree(S v, S* sh, Comb comb) {
   sh[threadIdx.x] = v;
   __syncthreads();
 #pragma unroll
@@ -229,7 +282,7 @@ __device__ __forceinline__ T block_tree(T v, T* sh, Comb comb) {
     if ((int)threadIdx.x < s) sh[threadIdx.x] = comb(sh[threadIdx.x], sh[threadIdx.x + s]);
     __syncthreads();
   }
-  T r = sh[0];
+  S r = sh[0];
   __syncthreads();
   return r;
 }
@@ -237,8 +290,8 @@ __device__ __forceinline__ T block_tree(T v, T* sh, Comb comb) {
 // Finish an extreme-value fold of output o from its summary a; all threads
 // of the workgroup call this together (uniform control flow) and thread 0
 // gets the result.
-template <class F>
-__device__ float ext_finish_block(const Geo& g, int64_t o, typename F::L a, typename F::L* sh) {
+template <class F, class T>
+__device__ T ext_finish_block(const Geo<T>& g, int64_t o, typename F::L a, typename F::L* sh) {
   const int64_t mid = g.mid;
   if (a.nan < 0) return F::pick(a);
   if (a.nan == mid - 1) return g.x[g.at(o, a.nan)];
@@ -249,8 +302,8 @@ __device__ float ext_finish_block(const Geo& g, int64_t o, typename F::L a, type
   t = block_tree(t, sh, [](typename F::L p, typename F::L q) { return F::comb(p, q); });
   return t.v;  // the tail holds no NaN and is not empty
 }
-template <class F>
-__device__ float ext_finish_thread(const Geo& g, int64_t o, const typename F::L& a) {
+template <class F, class T>
+__device__ T ext_finish_thread(const Geo<T>& g, int64_t o, const typename F::L& a) {
   const int64_t mid = g.mid;
   if (a.nan < 0) return F::pick(a);
   if (a.nan == mid - 1) return g.x[g.at(o, a.nan)];
@@ -262,30 +315,30 @@ __device__ float ext_finish_thread(const Geo& g, int64_t o, const typename F::L&
 
 // ------------------------------------------------------------ kernels
 // Run kernel: lo == 1; block b -> (run o = b / segs, segment s = b % segs).
-template <int OP, bool VEC>
-__global__ __launch_bounds__(kBlock) void fold_run_kernel(Geo g, int64_t segs, int64_t seglen, float* out,
+template <int OP, class T, bool VEC>
+__global__ __launch_bounds__(kBlock) void fold_run_kernel(Geo<T> g, int64_t segs, int64_t seglen, T* out,
                                                           Summ* summ) {
-  using F = typename Family<OP>::type;
+  using F = typename Family<OP, T>::type;
   using L = typename F::L;
+  using V = typename Vec16<T>::type;
+  constexpr int W = Vec16<T>::n;
   __shared__ L sh[kBlock];
   const int64_t b = blockIdx.x;
   const int64_t o = b / segs, s = b - o * segs;
   const int64_t m0 = s * seglen;
   const int64_t m1 = (m0 + seglen < g.mid) ? m0 + seglen : g.mid;
-  const float* run = g.x + o * g.mid;
+  const T* run = g.x + o * g.mid;
   L a = F::ident();
-  if (VEC) {  // run and segment start are 16-byte aligned, seglen % 4 == 0
-    const int64_t nv = (m1 - m0) / 4;
-    const f32x4* v = reinterpret_cast<const f32x4*>(run + m0);
+  if (VEC) {  // run and segment start are 16-byte aligned, seglen % W == 0
+    const int64_t nv = (m1 - m0) / W;
+    const V* v = reinterpret_cast<const V*>(run + m0);
     for (int64_t k = threadIdx.x; k < nv; k += kBlock) {
-      f32x4 q = __builtin_nontemporal_load(v + k);
-      const int64_t m = m0 + 4 * k;
-      F::add(a, q[0], m);
-      F::add(a, q[1], m + 1);
-      F::add(a, q[2], m + 2);
-      F::add(a, q[3], m + 3);
+      V q = __builtin_nontemporal_load(v + k);
+      const int64_t m = m0 + W * k;
+#pragma unroll
+      for (int e = 0; e < W; ++e) F::add(a, q[e], m + e);
     }
-    for (int64_t m = m0 + 4 * nv + threadIdx.x; m < m1; m += kBlock) F::add(a, run[m], m);
+    for (int64_t m = m0 + W * nv + threadIdx.x; m < m1; m += kBlock) F::add(a, run[m], m);
   } else {
     for (int64_t m = m0 + threadIdx.x; m < m1; m += kBlock) F::add(a, run[m], m);
   }
@@ -294,18 +347,18 @@ __global__ __launch_bounds__(kBlock) void fold_run_kernel(Geo g, int64_t segs, i
     if (threadIdx.x == 0) summ[o * segs + s] = F::summ(a);
     return;
   }
-  float r;
-  if constexpr (F::kRescan) r = ext_finish_block<F>(g, o, a, sh);
+  T r;
+  if constexpr (F::kRescan) r = ext_finish_block<F, T>(g, o, a, sh);
   else r = F::finish(F::summ(a), g.mid);
   if (threadIdx.x == 0) out[o] = r;
 }
 
 // Column kernel: one lane per (output, segment); consecutive lanes take
 // consecutive outputs of the same segment.
-template <int OP>
-__global__ __launch_bounds__(kBlock) void fold_col_kernel(Geo g, int64_t segs, int64_t seglen, float* out,
+template <int OP, class T>
+__global__ __launch_bounds__(kBlock) void fold_col_kernel(Geo<T> g, int64_t segs, int64_t seglen, T* out,
                                                           Summ* summ) {
-  using F = typename Family<OP>::type;
+  using F = typename Family<OP, T>::type;
   using L = typename F::L;
   const int64_t nout = g.lo * g.hi;
   const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -319,52 +372,56 @@ __global__ __launch_bounds__(kBlock) void fold_col_kernel(Geo g, int64_t segs, i
     summ[s * nout + o] = F::summ(a);  // segment-major: coalesced for the combine
     return;
   }
-  if constexpr (F::kRescan) out[o] = ext_finish_thread<F>(g, o, a);
+  if constexpr (F::kRescan) out[o] = ext_finish_thread<F, T>(g, o, a);
   else out[o] = F::finish(F::summ(a), g.mid);
 }
 
-// Column kernel, 16-byte form (lo % 4 == 0, x 16-byte aligned): one lane per
-// (group of 4 adjacent outputs, segment); each step along m loads one f32x4
-// holding element m of the 4 outputs, four steps at a time so that several
-// loads are in flight per lane.
-template <int OP>
-__global__ __launch_bounds__(kBlock) void fold_col4_kernel(Geo g, int64_t segs, int64_t seglen, float* out,
+// Column kernel, 16-byte form (lo % W == 0, x 16-byte aligned; W = 4 floats
+// or 2 doubles): one lane per (group of W adjacent outputs, segment); each
+// step along m loads one 16-byte vector holding element m of the W outputs,
+// four steps at a time so that several loads are in flight per lane.
+template <int OP, class T>
+__global__ __launch_bounds__(kBlock) void fold_colv_kernel(Geo<T> g, int64_t segs, int64_t seglen, T* out,
                                                            Summ* summ) {
-  using F = typename Family<OP>::type;
+  using F = typename Family<OP, T>::type;
   using L = typename F::L;
-  const int64_t nq = g.lo * g.hi / 4;
+  using V = typename Vec16<T>::type;
+  constexpr int W = Vec16<T>::n;
+  const int64_t nq = g.lo * g.hi / W;
   const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (t >= nq * segs) return;
   const int64_t s = t / nq, q = t - s * nq;
-  const int64_t o = 4 * q;  // first of the 4 outputs; the group never straddles h since lo % 4 == 0
+  const int64_t o = W * q;  // first of the W outputs; the group never straddles h since lo % W == 0
   const int64_t h = o / g.lo, i = o - h * g.lo;
-  const f32x4* p = reinterpret_cast<const f32x4*>(g.x + i + g.lo * g.mid * h);
-  const int64_t step = g.lo / 4;  // f32x4 elements between m and m + 1
+  const V* p = reinterpret_cast<const V*>(g.x + i + g.lo * g.mid * h);
+  const int64_t step = g.lo / W;  // vectors between m and m + 1
   const int64_t m0 = s * seglen;
   const int64_t m1 = (m0 + seglen < g.mid) ? m0 + seglen : g.mid;
-  L a[4] = {F::ident(), F::ident(), F::ident(), F::ident()};
+  L a[W];
+#pragma unroll
+  for (int k = 0; k < W; ++k) a[k] = F::ident();
   int64_t m = m0;
   for (; m + 4 <= m1; m += 4) {
-    f32x4 v[4];
+    V v[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) v[u] = __builtin_nontemporal_load(p + (m + u) * step);
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
 #pragma unroll
-      for (int k = 0; k < 4; ++k) F::add(a[k], v[u][k], m + u);
+      for (int k = 0; k < W; ++k) F::add(a[k], v[u][k], m + u);
     }
   }
   for (; m < m1; ++m) {
-    f32x4 v = __builtin_nontemporal_load(p + m * step);
+    V v = __builtin_nontemporal_load(p + m * step);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) F::add(a[k], v[k], m);
+    for (int k = 0; k < W; ++k) F::add(a[k], v[k], m);
   }
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
+  for (int k = 0; k < W; ++k) {
     if (segs > 1) {
       summ[s * (g.lo * g.hi) + o + k] = F::summ(a[k]);
     } else {
-      if constexpr (F::kRescan) out[o + k] = ext_finish_thread<F>(g, o + k, a[k]);
+      if constexpr (F::kRescan) out[o + k] = ext_finish_thread<F, T>(g, o + k, a[k]);
       else out[o + k] = F::finish(F::summ(a[k]), g.mid);
     }
   }
@@ -374,9 +431,9 @@ __global__ __launch_bounds__(kBlock) void fold_col4_kernel(Geo g, int64_t segs, 
 // of segments in order, the lanes then combine in a fixed tree (the sum and
 // product families are deterministic but not sequential in rounding; the
 // extreme-value summaries are order-free).
-template <int OP>
-__global__ __launch_bounds__(kBlock) void fold_combine_kernel(Geo g, int64_t segs, float* out, const Summ* summ) {
-  using F = typename Family<OP>::type;
+template <int OP, class T>
+__global__ __launch_bounds__(kBlock) void fold_combine_kernel(Geo<T> g, int64_t segs, T* out, const Summ* summ) {
+  using F = typename Family<OP, T>::type;
   __shared__ Summ sh[kBlock];
   const int64_t o = blockIdx.x;
   const int64_t per = (segs + kBlock - 1) / kBlock;
@@ -385,10 +442,10 @@ __global__ __launch_bounds__(kBlock) void fold_combine_kernel(Geo g, int64_t seg
   Summ a = F::sident();
   for (int64_t s = s0; s < s1; ++s) a = F::scomb(a, summ[o * segs + s]);
   a = block_tree(a, sh, [](Summ p, Summ q) { return F::scomb(p, q); });
-  float r;
+  T r;
   if constexpr (F::kRescan) {
     __shared__ typename F::L shl[kBlock];
-    r = ext_finish_block<F>(g, o, F::unsumm(a), shl);
+    r = ext_finish_block<F, T>(g, o, F::unsumm(a), shl);
   } else {
     r = F::finish(a, g.mid);
   }
@@ -398,31 +455,32 @@ __global__ __launch_bounds__(kBlock) void fold_combine_kernel(Geo g, int64_t seg
 // Combine kernel of the column paths: one lane per output folds its segment
 // summaries (segment-major, so neighbouring lanes read neighbouring
 // summaries) in segment order.
-template <int OP>
-__global__ __launch_bounds__(kBlock) void fold_col_combine_kernel(Geo g, int64_t segs, float* out,
+template <int OP, class T>
+__global__ __launch_bounds__(kBlock) void fold_col_combine_kernel(Geo<T> g, int64_t segs, T* out,
                                                                   const Summ* summ) {
-  using F = typename Family<OP>::type;
+  using F = typename Family<OP, T>::type;
   const int64_t nout = g.lo * g.hi;
   const int64_t o = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (o >= nout) return;
   Summ a = summ[o];
   for (int64_t s = 1; s < segs; ++s) a = F::scomb(a, summ[s * nout + o]);
-  if constexpr (F::kRescan) out[o] = ext_finish_thread<F>(g, o, F::unsumm(a));
+  if constexpr (F::kRescan) out[o] = ext_finish_thread<F, T>(g, o, F::unsumm(a));
   else out[o] = F::finish(a, g.mid);
 }
 
 // Output for an empty middle extent: NT2 folds nothing from the neutral
 // (mean: rec(0) * 0 = NaN).
-__global__ void fold_fill_kernel(float* out, int64_t n, float v) {
+template <class T> __global__ void fold_fill_kernel(T* out, int64_t n, T v) {
   int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i < n) out[i] = v;
 }
 
-template <int OP>
-int fold_launch(const float* x, int64_t lo, int64_t mid, int64_t hi, float* out, nt2h_stream_t st) {
+template <int OP, class T>
+int fold_launch(const T* x, int64_t lo, int64_t mid, int64_t hi, T* out, nt2h_stream_t st) {
+  constexpr int W = Vec16<T>::n;
   hipStream_t s = as_hip(st);
   const int64_t nout = lo * hi;
-  Geo g{x, lo, mid, hi};
+  Geo<T> g{x, lo, mid, hi};
   if (lo == 1 && mid >= 512) {
     // enough workgroups to fill the chip; segments of at least 2048 elements
     int64_t segs = (8192 + hi - 1) / hi;
@@ -430,26 +488,26 @@ int fold_launch(const float* x, int64_t lo, int64_t mid, int64_t hi, float* out,
     if (segs > maxsegs) segs = maxsegs;
     if (segs < 1) segs = 1;
     int64_t seglen = (mid + segs - 1) / segs;
-    seglen = (seglen + 3) / 4 * 4;
+    seglen = (seglen + W - 1) / W * W;
     segs = (mid + seglen - 1) / seglen;
-    const bool vec = (mid % 4 == 0) && aligned16(x);
+    const bool vec = (mid % W == 0) && aligned16(x);
     Summ* summ = nullptr;
     if (segs > 1) NT2H_TRY(get_scratch(st, (size_t)(nout * segs) * sizeof(Summ), (void**)&summ));
     const int64_t blocks = nout * segs;
     if (blocks > 0x7fffffffLL) return NT2H_ERR_TOO_LARGE;
-    if (vec) hipLaunchKernelGGL((fold_run_kernel<OP, true>), dim3((unsigned)blocks), dim3(kBlock), 0, s, g, segs, seglen, out, summ);
-    else hipLaunchKernelGGL((fold_run_kernel<OP, false>), dim3((unsigned)blocks), dim3(kBlock), 0, s, g, segs, seglen, out, summ);
+    if (vec) hipLaunchKernelGGL((fold_run_kernel<OP, T, true>), dim3((unsigned)blocks), dim3(kBlock), 0, s, g, segs, seglen, out, summ);
+    else hipLaunchKernelGGL((fold_run_kernel<OP, T, false>), dim3((unsigned)blocks), dim3(kBlock), 0, s, g, segs, seglen, out, summ);
     NT2H_HIP(hipGetLastError());
     if (segs > 1) {
-      hipLaunchKernelGGL((fold_combine_kernel<OP>), dim3((unsigned)nout), dim3(kBlock), 0, s, g, segs, out,
+      hipLaunchKernelGGL((fold_combine_kernel<OP, T>), dim3((unsigned)nout), dim3(kBlock), 0, s, g, segs, out,
                          (const Summ*)summ);
       NT2H_HIP(hipGetLastError());
     }
     return NT2H_OK;
   }
   // column kernel: aim for >= 2^18 lanes; segments of at least 16 elements
-  const bool col4 = (lo % 4 == 0) && aligned16(x);
-  const int64_t units = col4 ? nout / 4 : nout;  // lanes per segment
+  const bool colv = (lo % W == 0) && aligned16(x);
+  const int64_t units = colv ? nout / W : nout;  // lanes per segment
   int64_t segs = ((int64_t)1 << 18) / units;
   int64_t maxsegs = (mid + 15) / 16;
   if (segs > maxsegs) segs = maxsegs;
@@ -465,26 +523,55 @@ int fold_launch(const float* x, int64_t lo, int64_t mid, int64_t hi, float* out,
   const int64_t lanes = units * segs;
   const int64_t blocks = (lanes + kBlock - 1) / kBlock;
   if (blocks > 0x7fffffffLL) return NT2H_ERR_TOO_LARGE;
-  if (col4) hipLaunchKernelGGL((fold_col4_kernel<OP>), dim3((unsigned)blocks), dim3(kBlock), 0, s, g, segs, seglen, out, summ);
-  else hipLaunchKernelGGL((fold_col_kernel<OP>), dim3((unsigned)blocks), dim3(kBlock), 0, s, g, segs, seglen, out, summ);
+  if (colv) hipLaunchKernelGGL((fold_colv_kernel<OP, T>), dim3((unsigned)blocks), dim3(kBlock), 0, s, g, segs, seglen, out, summ);
+  else hipLaunchKernelGGL((fold_col_kernel<OP, T>), dim3((unsigned)blocks), dim3(kBlock), 0, s, g, segs, seglen, out, summ);
   NT2H_HIP(hipGetLastError());
   if (segs > 1) {
     const int64_t cblocks = (nout + kBlock - 1) / kBlock;
-    hipLaunchKernelGGL((fold_col_combine_kernel<OP>), dim3((unsigned)cblocks), dim3(kBlock), 0, s, g, segs, out,
+    hipLaunchKernelGGL((fold_col_combine_kernel<OP, T>), dim3((unsigned)cblocks), dim3(kBlock), 0, s, g, segs, out,
                        (const Summ*)summ);
     NT2H_HIP(hipGetLastError());
   }
   return NT2H_OK;
 }
 
-inline float fold_neutral_result(int op) {
+template <class T> inline T fold_neutral_result(int op) {
+  const bool dbl = sizeof(T) == 8;
   switch (op) {
-    case NT2H_FOLD_PROD: return 1.0f;
-    case NT2H_FOLD_MAXIMUM: return -0x1.fffffep+127f;
-    case NT2H_FOLD_MINIMUM: return 0x1.fffffep+127f;
-    case NT2H_FOLD_MEAN: return __builtin_nanf("");
-    default: return 0.0f;
+    case NT2H_FOLD_PROD: return T(1);
+    case NT2H_FOLD_MAXIMUM: return dbl ? (T)-0x1.fffffffffffffp+1023 : (T)-0x1.fffffep+127f;
+    case NT2H_FOLD_MINIMUM: return dbl ? (T)0x1.fffffffffffffp+1023 : (T)0x1.fffffep+127f;
+    case NT2H_FOLD_MEAN: return (T)__builtin_nan("");
+    default: return T(0);
   }
+}
+
+template <class T>
+int fold_entry(int op, const T* x, int64_t lo, int64_t mid, int64_t hi, T* out, nt2h_stream_t s) {
+  if (op < NT2H_FOLD_SUM || op > NT2H_FOLD_MEAN) return NT2H_ERR_BAD_OP;
+  if (lo < 0 || mid < 0 || hi < 0) return NT2H_ERR_INVALID_ARG;
+  if (lo > 0 && hi > 0 && (mid > INT64_MAX / lo || lo * mid > INT64_MAX / hi)) return NT2H_ERR_TOO_LARGE;
+  const int64_t nout = lo * hi;
+  if (nout == 0) return NT2H_OK;
+  if (!out || (mid > 0 && !x)) return NT2H_ERR_NULL_POINTER;
+  if (mid == 0) {
+    unsigned blocks = (unsigned)((nout + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(fold_fill_kernel<T>, dim3(blocks), dim3(kBlock), 0, as_hip(s), out, nout,
+                       fold_neutral_result<T>(op));
+    NT2H_HIP(hipGetLastError());
+    return NT2H_OK;
+  }
+  switch (op) {
+    case NT2H_FOLD_SUM: return fold_launch<F_SUM, T>(x, lo, mid, hi, out, s);
+    case NT2H_FOLD_PROD: return fold_launch<F_PROD, T>(x, lo, mid, hi, out, s);
+    case NT2H_FOLD_MAXIMUM: return fold_launch<F_MAX, T>(x, lo, mid, hi, out, s);
+    case NT2H_FOLD_MINIMUM: return fold_launch<F_MIN, T>(x, lo, mid, hi, out, s);
+    case NT2H_FOLD_ASUM1: return fold_launch<F_ASUM1, T>(x, lo, mid, hi, out, s);
+    case NT2H_FOLD_ASUM2: return fold_launch<F_ASUM2, T>(x, lo, mid, hi, out, s);
+    case NT2H_FOLD_NORM2: return fold_launch<F_NORM2, T>(x, lo, mid, hi, out, s);
+    case NT2H_FOLD_MEAN: return fold_launch<F_MEAN, T>(x, lo, mid, hi, out, s);
+  }
+  return NT2H_ERR_BAD_OP;
 }
 
 }  // namespace nt2h
@@ -494,29 +581,10 @@ using namespace nt2h;
 extern "C" {
 
 int nt2h_fold_f32(int op, const float* x, int64_t lo, int64_t mid, int64_t hi, float* out, nt2h_stream_t s) {
-  if (op < NT2H_FOLD_SUM || op > NT2H_FOLD_MEAN) return NT2H_ERR_BAD_OP;
-  if (lo < 0 || mid < 0 || hi < 0) return NT2H_ERR_INVALID_ARG;
-  if (lo > 0 && hi > 0 && (mid > INT64_MAX / lo || lo * mid > INT64_MAX / hi)) return NT2H_ERR_TOO_LARGE;
-  const int64_t nout = lo * hi;
-  if (nout == 0) return NT2H_OK;
-  if (!out || (mid > 0 && !x)) return NT2H_ERR_NULL_POINTER;
-  if (mid == 0) {
-    unsigned blocks = (unsigned)((nout + kBlock - 1) / kBlock);
-    hipLaunchKernelGGL(fold_fill_kernel, dim3(blocks), dim3(kBlock), 0, as_hip(s), out, nout, fold_neutral_result(op));
-    NT2H_HIP(hipGetLastError());
-    return NT2H_OK;
-  }
-  switch (op) {
-    case NT2H_FOLD_SUM: return fold_launch<F_SUM>(x, lo, mid, hi, out, s);
-    case NT2H_FOLD_PROD: return fold_launch<F_PROD>(x, lo, mid, hi, out, s);
-    case NT2H_FOLD_MAXIMUM: return fold_launch<F_MAX>(x, lo, mid, hi, out, s);
-    case NT2H_FOLD_MINIMUM: return fold_launch<F_MIN>(x, lo, mid, hi, out, s);
-    case NT2H_FOLD_ASUM1: return fold_launch<F_ASUM1>(x, lo, mid, hi, out, s);
-    case NT2H_FOLD_ASUM2: return fold_launch<F_ASUM2>(x, lo, mid, hi, out, s);
-    case NT2H_FOLD_NORM2: return fold_launch<F_NORM2>(x, lo, mid, hi, out, s);
-    case NT2H_FOLD_MEAN: return fold_launch<F_MEAN>(x, lo, mid, hi, out, s);
-  }
-  return NT2H_ERR_BAD_OP;
+  return fold_entry<float>(op, x, lo, mid, hi, out, s);
+}
+int nt2h_fold_f64(int op, const double* x, int64_t lo, int64_t mid, int64_t hi, double* out, nt2h_stream_t s) {
+  return fold_entry<double>(op, x, lo, mid, hi, out, s);
 }
 
 }  // extern "C"

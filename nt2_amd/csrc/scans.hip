@@ -1,5 +1,5 @@
 // Scans of NT2 (cumsum, cumprod, cummax, cummin) over one dimension of a
-// contiguous table: cumulative_<tag, bop, neutral> with plus/Zero,
+// contiguous float or double table: cumulative_<tag, bop, neutral> with plus/Zero,
 // multiplies/One, max/Minf, min/Inf (boost/simd/swar/functions/{cumsum,
 // cumprod,cummax,cummin}.hpp) run by inner_scan / outer_scan over the middle
 // index of the (lo, mid, hi) reshaping, with the scalar recurrence
@@ -14,18 +14,19 @@
 //    L.acc : R.acc) — associative, and each output equals NT2's recurrence
 //    bit for bit (a NaN resets the running value, ties keep the later
 //    element, as in the scalar a0 > a1 ? a0 : a1);
-//  * cumsum: TwoSum-compensated float pairs inside a tile, FP64 carries
-//    across tiles and segments (more accurate than NT2's sequential float
-//    recurrence, so not bit-identical to it);
+//  * cumsum: TwoSum-compensated pairs in the table's type inside a tile,
+//    compensated FP64 pairs carried across tiles and segments (more accurate
+//    than NT2's sequential recurrence, so not bit-identical to it);
 //  * cumprod: an FP64 mantissa with a separate integer exponent, rounded to
-//    float per output (no intermediate overflow).
+//    the table's type per output (no intermediate overflow).
 // Kernels:
 //  * run kernel (lo == 1, mid >= 512): one workgroup per (run, segment)
-//    walks the segment in tiles of 1024 elements (lane l holds elements
-//    4l..4l+3), scans each tile in the workgroup (wave shuffles, then the
-//    four wave totals through LDS) and carries the tile total forward;
+//    walks the segment in tiles of 256 x W elements (W = 4 floats or 2
+//    doubles per 16 bytes; lane l holds elements W*l..W*l+W-1), scans each
+//    tile in the workgroup (wave shuffles, then the four wave totals
+//    through LDS) and carries the tile total forward;
 //  * column kernel (lo > 1, or short runs): one lane per (output, segment)
-//    running the recurrence — or, when lo % 4 == 0, per group of 4 adjacent
+//    running the recurrence — or, when lo % W == 0, per group of W adjacent
 //    outputs with 16-byte loads and stores; neighbouring lanes touch
 //    neighbouring addresses;
 //  * long runs split into segments: pass 1 computes each segment's total,
@@ -38,9 +39,9 @@
 
 namespace nt2h {
 
-struct SGeo {
-  const float* x;
-  float* y;
+template <class T> struct SGeo {
+  const T* x;
+  T* y;
   int64_t lo, mid, hi;
   __device__ __forceinline__ int64_t at(int64_t o, int64_t m) const {
     if (lo == 1) return o * mid + m;
@@ -52,31 +53,48 @@ struct SSumm {
   double w[2];
 };
 
+template <class T> __device__ __forceinline__ void two_sum_s(T a, T b, T& s, T& e) {
+  s = a + b;
+  T bp = s - a;
+  e = (a - (s - bp)) + (b - bp);
+}
+__device__ __forceinline__ float shfl_up_t(float v, int d) { return __shfl_up(v, d, 64); }
+__device__ __forceinline__ double shfl_up_t(double v, int d) { return __shfl_up(v, d, 64); }
+__device__ __forceinline__ int shfl_up_t(int v, int d) { return __shfl_up(v, d, 64); }
+
 // ------------------------------------------------------------- cumsum
-struct MSum {
+template <class T> struct MSum {
   struct V {
-    float s, c;
+    T s, c;
   };
-  using C = double;
-  __device__ static V ident() { return V{0.0f, 0.0f}; }
-  __device__ static V lift(float x) { return V{x, 0.0f}; }
+  struct C {
+    double s, c;  // compensated FP64 carry
+  };
+  __device__ static V ident() { return V{T(0), T(0)}; }
+  __device__ static V lift(T x) { return V{x, T(0)}; }
   __device__ static V comb(const V& a, const V& b) {
-    float t, e;
-    two_sum(a.s, b.s, t, e);
+    T t, e;
+    two_sum_s(a.s, b.s, t, e);
     return V{t, (a.c + b.c) + e};
   }
-  __device__ static V shfl_up(const V& v, int d) { return V{__shfl_up(v.s, d, 64), __shfl_up(v.c, d, 64)}; }
-  __device__ static V shfl(const V& v, int l) { return V{__shfl(v.s, l, 64), __shfl(v.c, l, 64)}; }
-  __device__ static C cident() { return 0.0; }
-  __device__ static C cadd(C c, const V& v) { return c + ((double)v.s + (double)v.c); }
-  __device__ static C ccomb(C a, C b) { return a + b; }
-  __device__ static float fin(C c, const V& v) { return (float)(c + ((double)v.s + (double)v.c)); }
-  __device__ static SSumm pack(C c) { return SSumm{{c, 0.0}}; }
-  __device__ static C unpack(const SSumm& s) { return s.w[0]; }
+  __device__ static V shfl_up(const V& v, int d) { return V{shfl_up_t(v.s, d), shfl_up_t(v.c, d)}; }
+  __device__ static C cident() { return C{0.0, 0.0}; }
+  __device__ static C ccomb(const C& a, const C& b) {
+    double t, e;
+    two_sum_s(a.s, b.s, t, e);
+    return C{t, (a.c + b.c) + e};
+  }
+  __device__ static C cadd(const C& c, const V& v) { return ccomb(c, C{(double)v.s, (double)v.c}); }
+  __device__ static T fin(const C& c, const V& v) {
+    C r = cadd(c, v);
+    return (T)(r.s + r.c);
+  }
+  __device__ static SSumm pack(const C& c) { return SSumm{{c.s, c.c}}; }
+  __device__ static C unpack(const SSumm& s) { return C{s.w[0], s.w[1]}; }
 };
 
 // ------------------------------------------------------------ cumprod
-struct MProd {
+template <class T> struct MProd {
   struct V {
     double m;
     int e;
@@ -92,53 +110,51 @@ struct MProd {
     return a;
   }
   __device__ static V ident() { return V{1.0, 0}; }
-  __device__ static V lift(float x) { return norm(V{(double)x, 0}); }
+  __device__ static V lift(T x) { return norm(V{(double)x, 0}); }
   __device__ static V comb(const V& a, const V& b) { return norm(V{a.m * b.m, a.e + b.e}); }
-  __device__ static V shfl_up(const V& v, int d) { return V{__shfl_up(v.m, d, 64), __shfl_up(v.e, d, 64)}; }
-  __device__ static V shfl(const V& v, int l) { return V{__shfl(v.m, l, 64), __shfl(v.e, l, 64)}; }
+  __device__ static V shfl_up(const V& v, int d) { return V{shfl_up_t(v.m, d), shfl_up_t(v.e, d)}; }
   __device__ static C cident() { return ident(); }
   __device__ static C cadd(const C& c, const V& v) { return comb(c, v); }
   __device__ static C ccomb(const C& a, const C& b) { return comb(a, b); }
-  __device__ static float fin(const C& c, const V& v) {
+  __device__ static T fin(const C& c, const V& v) {
     V r = comb(c, v);
-    if (r.m == 0.0 || !__builtin_isfinite(r.m)) return (float)r.m;
+    if (r.m == 0.0 || !__builtin_isfinite(r.m)) return (T)r.m;
     int e = r.e > 2000 ? 2000 : (r.e < -2000 ? -2000 : r.e);
-    return (float)__builtin_ldexp(r.m, e);
+    return (T)__builtin_ldexp(r.m, e);
   }
   __device__ static SSumm pack(const C& c) { return SSumm{{c.m, (double)c.e}}; }
   __device__ static C unpack(const SSumm& s) { return C{s.w[0], (int)s.w[1]}; }
 };
 
 // ---------------------------------------------------- cummax / cummin
-template <bool MAX> struct MExt {
+template <bool MAX, class T> struct MExt {
   struct V {
-    int nan;    // the covered elements contain a NaN
-    float acc;  // NT2's recurrence over them, started from the neutral
+    int nan;  // the covered elements contain a NaN
+    T acc;    // NT2's recurrence over them, started from the neutral
   };
   using C = V;
-  __device__ static float neutral() { return MAX ? -__builtin_inff() : __builtin_inff(); }
+  __device__ static T neutral() { return MAX ? -(T)__builtin_inf() : (T)__builtin_inf(); }
   __device__ static V ident() { return V{0, neutral()}; }
-  __device__ static V lift(float x) { return V{x != x ? 1 : 0, x}; }
+  __device__ static V lift(T x) { return V{x != x ? 1 : 0, x}; }
   __device__ static V comb(const V& a, const V& b) {
     if (b.nan) return b;
     const bool keep_a = MAX ? (a.acc > b.acc) : (a.acc < b.acc);
     return V{a.nan, keep_a ? a.acc : b.acc};
   }
-  __device__ static V shfl_up(const V& v, int d) { return V{__shfl_up(v.nan, d, 64), __shfl_up(v.acc, d, 64)}; }
-  __device__ static V shfl(const V& v, int l) { return V{__shfl(v.nan, l, 64), __shfl(v.acc, l, 64)}; }
+  __device__ static V shfl_up(const V& v, int d) { return V{shfl_up_t(v.nan, d), shfl_up_t(v.acc, d)}; }
   __device__ static C cident() { return ident(); }
   __device__ static C cadd(const C& c, const V& v) { return comb(c, v); }
   __device__ static C ccomb(const C& a, const C& b) { return comb(a, b); }
-  __device__ static float fin(const C& c, const V& v) { return comb(c, v).acc; }
+  __device__ static T fin(const C& c, const V& v) { return comb(c, v).acc; }
   __device__ static SSumm pack(const C& c) { return SSumm{{(double)c.nan, (double)c.acc}}; }
-  __device__ static C unpack(const SSumm& s) { return C{(int)s.w[0], (float)s.w[1]}; }
+  __device__ static C unpack(const SSumm& s) { return C{(int)s.w[0], (T)s.w[1]}; }
 };
 
-template <int OP> struct SFam;
-template <> struct SFam<NT2H_SCAN_CUMSUM> { using type = MSum; };
-template <> struct SFam<NT2H_SCAN_CUMPROD> { using type = MProd; };
-template <> struct SFam<NT2H_SCAN_CUMMAX> { using type = MExt<true>; };
-template <> struct SFam<NT2H_SCAN_CUMMIN> { using type = MExt<false>; };
+template <int OP, class T> struct SFam;
+template <class T> struct SFam<NT2H_SCAN_CUMSUM, T> { using type = MSum<T>; };
+template <class T> struct SFam<NT2H_SCAN_CUMPROD, T> { using type = MProd<T>; };
+template <class T> struct SFam<NT2H_SCAN_CUMMAX, T> { using type = MExt<true, T>; };
+template <class T> struct SFam<NT2H_SCAN_CUMMIN, T> { using type = MExt<false, T>; };
 
 // Ordered exclusive scan of one value per lane over the workgroup; returns
 // the exclusive prefix of this lane and the workgroup total in *total.
@@ -172,30 +188,30 @@ __device__ __forceinline__ typename M::V block_exscan(typename M::V v, typename 
 
 // Run kernel body: scan elements [m0, m1) of run o starting from carry;
 // WRITE = false computes only the total (pass 1 of segmented scans).
-template <class M, bool VEC, bool WRITE>
-__device__ typename M::C run_segment(const SGeo& g, int64_t o, int64_t m0, int64_t m1, typename M::C carry) {
+template <class M, class T, bool VEC, bool WRITE>
+__device__ typename M::C run_segment(const SGeo<T>& g, int64_t o, int64_t m0, int64_t m1, typename M::C carry) {
   using V = typename M::V;
+  using VT = typename Vec16<T>::type;
+  constexpr int W = Vec16<T>::n;
   __shared__ V lds[kWaves];
-  const float* run = g.x + o * g.mid;
-  float* out = g.y + o * g.mid;
+  const T* run = g.x + o * g.mid;
+  T* out = g.y + o * g.mid;
   if (VEC) {
-    for (int64_t base = m0; base < m1; base += 4 * kBlock) {
-      const int64_t m = base + 4 * (int64_t)threadIdx.x;
-      float e[4];
-      if (m + 3 < m1) {
-        f32x4 q = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(run + m));
-        e[0] = q[0];
-        e[1] = q[1];
-        e[2] = q[2];
-        e[3] = q[3];
+    for (int64_t base = m0; base < m1; base += W * kBlock) {
+      const int64_t m = base + W * (int64_t)threadIdx.x;
+      T e[W];
+      if (m + W - 1 < m1) {
+        VT q = __builtin_nontemporal_load(reinterpret_cast<const VT*>(run + m));
+#pragma unroll
+        for (int k = 0; k < W; ++k) e[k] = q[k];
       } else {
 #pragma unroll
-        for (int k = 0; k < 4; ++k) e[k] = (m + k < m1) ? run[m + k] : 0.0f;
+        for (int k = 0; k < W; ++k) e[k] = (m + k < m1) ? run[m + k] : T(0);
       }
-      V p[4];
+      V p[W];
       V acc = M::ident();
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
+      for (int k = 0; k < W; ++k) {
         V xk = (m + k < m1) ? M::lift(e[k]) : M::ident();
         acc = (k == 0) ? xk : M::comb(acc, xk);
         p[k] = acc;
@@ -203,15 +219,17 @@ __device__ typename M::C run_segment(const SGeo& g, int64_t o, int64_t m0, int64
       V tot;
       V ex = block_exscan<M>(acc, lds, &tot);
       if (WRITE) {
-        float r[4];
+        T r[W];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) r[k] = M::fin(carry, M::comb(ex, p[k]));
-        if (m + 3 < m1) {
-          f32x4 q = {r[0], r[1], r[2], r[3]};
-          __builtin_nontemporal_store(q, reinterpret_cast<f32x4*>(out + m));
+        for (int k = 0; k < W; ++k) r[k] = M::fin(carry, M::comb(ex, p[k]));
+        if (m + W - 1 < m1) {
+          VT q;
+#pragma unroll
+          for (int k = 0; k < W; ++k) q[k] = r[k];
+          __builtin_nontemporal_store(q, reinterpret_cast<VT*>(out + m));
         } else {
 #pragma unroll
-          for (int k = 0; k < 4; ++k)
+          for (int k = 0; k < W; ++k)
             if (m + k < m1) out[m + k] = r[k];
         }
       }
@@ -232,27 +250,27 @@ __device__ typename M::C run_segment(const SGeo& g, int64_t o, int64_t m0, int64
 
 // lo == 1: block b -> (run o = b / segs, segment s).  PASS 0: single pass
 // (segs == 1); PASS 1: segment totals; PASS 3: rescan from the prefixes.
-template <int OP, bool VEC, int PASS>
-__global__ __launch_bounds__(kBlock) void scan_run_kernel(SGeo g, int64_t segs, int64_t seglen, SSumm* summ) {
-  using M = typename SFam<OP>::type;
+template <int OP, class T, bool VEC, int PASS>
+__global__ __launch_bounds__(kBlock) void scan_run_kernel(SGeo<T> g, int64_t segs, int64_t seglen, SSumm* summ) {
+  using M = typename SFam<OP, T>::type;
   const int64_t b = blockIdx.x;
   const int64_t o = b / segs, s = b - o * segs;
   const int64_t m0 = s * seglen;
   const int64_t m1 = (m0 + seglen < g.mid) ? m0 + seglen : g.mid;
   if (PASS == 1) {
-    typename M::C t = run_segment<M, VEC, false>(g, o, m0, m1, M::cident());
+    typename M::C t = run_segment<M, T, VEC, false>(g, o, m0, m1, M::cident());
     if (threadIdx.x == 0) summ[o * segs + s] = M::pack(t);
   } else {
     typename M::C c = (PASS == 3) ? M::unpack(summ[o * segs + s]) : M::cident();
-    run_segment<M, VEC, true>(g, o, m0, m1, c);
+    run_segment<M, T, VEC, true>(g, o, m0, m1, c);
   }
 }
 
 // Column kernel: lane per (output, segment), the recurrence run in order.
 // PASS as above.
-template <int OP, int PASS>
-__global__ __launch_bounds__(kBlock) void scan_col_kernel(SGeo g, int64_t segs, int64_t seglen, SSumm* summ) {
-  using M = typename SFam<OP>::type;
+template <int OP, class T, int PASS>
+__global__ __launch_bounds__(kBlock) void scan_col_kernel(SGeo<T> g, int64_t segs, int64_t seglen, SSumm* summ) {
+  using M = typename SFam<OP, T>::type;
   const int64_t nout = g.lo * g.hi;
   const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (t >= nout * segs) return;
@@ -269,43 +287,45 @@ __global__ __launch_bounds__(kBlock) void scan_col_kernel(SGeo g, int64_t segs, 
   if (PASS == 1) summ[s * nout + o] = M::pack(M::cadd(c, acc));
 }
 
-// Column kernel, 16-byte form (lo % 4 == 0, x and y 16-byte aligned): one
-// lane per (group of 4 adjacent outputs, segment), one f32x4 load and store
-// per step along m, loads issued four steps ahead.
-template <int OP, int PASS>
-__global__ __launch_bounds__(kBlock) void scan_col4_kernel(SGeo g, int64_t segs, int64_t seglen, SSumm* summ) {
-  using M = typename SFam<OP>::type;
+// Column kernel, 16-byte form (lo % W == 0, x and y 16-byte aligned): one
+// lane per (group of W adjacent outputs, segment), one 16-byte load and
+// store per step along m, loads issued four steps ahead.
+template <int OP, class T, int PASS>
+__global__ __launch_bounds__(kBlock) void scan_colv_kernel(SGeo<T> g, int64_t segs, int64_t seglen, SSumm* summ) {
+  using M = typename SFam<OP, T>::type;
   using V = typename M::V;
   using C = typename M::C;
-  const int64_t nq = g.lo * g.hi / 4;
+  using VT = typename Vec16<T>::type;
+  constexpr int W = Vec16<T>::n;
+  const int64_t nq = g.lo * g.hi / W;
   const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (t >= nq * segs) return;
   const int64_t s = t / nq, q = t - s * nq;
-  const int64_t o = 4 * q;
+  const int64_t o = W * q;
   const int64_t h = o / g.lo, i = o - h * g.lo;
   const int64_t off = i + g.lo * g.mid * h;
-  const f32x4* px = reinterpret_cast<const f32x4*>(g.x + off);
-  f32x4* py = reinterpret_cast<f32x4*>(g.y + off);
-  const int64_t step = g.lo / 4;
+  const VT* px = reinterpret_cast<const VT*>(g.x + off);
+  VT* py = reinterpret_cast<VT*>(g.y + off);
+  const int64_t step = g.lo / W;
   const int64_t m0 = s * seglen;
   const int64_t m1 = (m0 + seglen < g.mid) ? m0 + seglen : g.mid;
-  C c[4];
-  V acc[4];
+  C c[W];
+  V acc[W];
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
+  for (int k = 0; k < W; ++k) {
     c[k] = (PASS == 3) ? M::unpack(summ[s * (g.lo * g.hi) + o + k]) : M::cident();
     acc[k] = M::ident();
   }
   int64_t m = m0;
   for (; m + 4 <= m1; m += 4) {
-    f32x4 v[4];
+    VT v[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) v[u] = __builtin_nontemporal_load(px + (m + u) * step);
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      f32x4 r;
+      VT r;
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
+      for (int k = 0; k < W; ++k) {
         acc[k] = M::comb(acc[k], M::lift(v[u][k]));
         r[k] = M::fin(c[k], acc[k]);
       }
@@ -313,10 +333,10 @@ __global__ __launch_bounds__(kBlock) void scan_col4_kernel(SGeo g, int64_t segs,
     }
   }
   for (; m < m1; ++m) {
-    f32x4 v = __builtin_nontemporal_load(px + m * step);
-    f32x4 r;
+    VT v = __builtin_nontemporal_load(px + m * step);
+    VT r;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < W; ++k) {
       acc[k] = M::comb(acc[k], M::lift(v[k]));
       r[k] = M::fin(c[k], acc[k]);
     }
@@ -324,15 +344,15 @@ __global__ __launch_bounds__(kBlock) void scan_col4_kernel(SGeo g, int64_t segs,
   }
   if (PASS == 1) {
 #pragma unroll
-    for (int k = 0; k < 4; ++k) summ[s * (g.lo * g.hi) + o + k] = M::pack(M::cadd(c[k], acc[k]));
+    for (int k = 0; k < W; ++k) summ[s * (g.lo * g.hi) + o + k] = M::pack(M::cadd(c[k], acc[k]));
   }
 }
 
 // Pass 2 of the column paths: one lane per output replaces its segment
 // totals (segment-major) by their exclusive prefixes, in segment order.
-template <int OP>
+template <int OP, class T>
 __global__ __launch_bounds__(kBlock) void scan_col_prefix_kernel(int64_t nout, int64_t segs, SSumm* summ) {
-  using M = typename SFam<OP>::type;
+  using M = typename SFam<OP, T>::type;
   using C = typename M::C;
   const int64_t o = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (o >= nout) return;
@@ -346,9 +366,9 @@ __global__ __launch_bounds__(kBlock) void scan_col_prefix_kernel(int64_t nout, i
 
 // Pass 2: per output (one workgroup), replace the segment totals by their
 // exclusive prefixes, in segment order.
-template <int OP>
+template <int OP, class T>
 __global__ __launch_bounds__(kBlock) void scan_prefix_kernel(int64_t segs, SSumm* summ) {
-  using M = typename SFam<OP>::type;
+  using M = typename SFam<OP, T>::type;
   using C = typename M::C;
   __shared__ SSumm sh[kBlock];
   const int64_t o = blockIdx.x;
@@ -378,42 +398,47 @@ __global__ __launch_bounds__(kBlock) void scan_prefix_kernel(int64_t segs, SSumm
   }
 }
 
-template <int OP>
-int scan_launch(const float* x, float* y, int64_t lo, int64_t mid, int64_t hi, nt2h_stream_t st) {
+template <int OP, class T>
+int scan_launch(const T* x, T* y, int64_t lo, int64_t mid, int64_t hi, nt2h_stream_t st) {
+  constexpr int W = Vec16<T>::n;
   hipStream_t s = as_hip(st);
   const int64_t nout = lo * hi;
-  SGeo g{x, y, lo, mid, hi};
+  SGeo<T> g{x, y, lo, mid, hi};
   if (lo == 1 && mid >= 512) {
+    const int64_t tile = (int64_t)W * kBlock;
     int64_t segs = (8192 + hi - 1) / hi;
-    int64_t maxsegs = (mid + 8191) / 8192;  // segments of at least 8 tiles
+    int64_t maxsegs = (mid + 8 * tile - 1) / (8 * tile);  // segments of at least 8 tiles
     if (segs > maxsegs) segs = maxsegs;
     if (segs < 1) segs = 1;
     int64_t seglen = (mid + segs - 1) / segs;
-    seglen = (seglen + 4 * kBlock - 1) / (4 * kBlock) * (4 * kBlock);
+    seglen = (seglen + tile - 1) / tile * tile;
     segs = (mid + seglen - 1) / seglen;
-    const bool vec = (mid % 4 == 0) && aligned16(x, y);
+    const bool vec = (mid % W == 0) && aligned16(x, y);
     const int64_t blocks = nout * segs;
     if (blocks > 0x7fffffffLL) return NT2H_ERR_TOO_LARGE;
     SSumm* summ = nullptr;
+#define NT2H_SCAN_RUN(PASS)                                                                                      \
+  do {                                                                                                            \
+    if (vec) hipLaunchKernelGGL((scan_run_kernel<OP, T, true, PASS>), dim3((unsigned)blocks), dim3(kBlock), 0,  \
+                                s, g, segs, seglen, summ);                                                        \
+    else hipLaunchKernelGGL((scan_run_kernel<OP, T, false, PASS>), dim3((unsigned)blocks), dim3(kBlock), 0, s, \
+                            g, segs, seglen, summ);                                                               \
+    NT2H_HIP(hipGetLastError());                                                                                  \
+  } while (0)
     if (segs == 1) {
-      if (vec) hipLaunchKernelGGL((scan_run_kernel<OP, true, 0>), dim3((unsigned)blocks), dim3(kBlock), 0, s, g, segs, seglen, summ);
-      else hipLaunchKernelGGL((scan_run_kernel<OP, false, 0>), dim3((unsigned)blocks), dim3(kBlock), 0, s, g, segs, seglen, summ);
-      NT2H_HIP(hipGetLastError());
+      NT2H_SCAN_RUN(0);
       return NT2H_OK;
     }
     NT2H_TRY(get_scratch(st, (size_t)blocks * sizeof(SSumm), (void**)&summ));
-    if (vec) hipLaunchKernelGGL((scan_run_kernel<OP, true, 1>), dim3((unsigned)blocks), dim3(kBlock), 0, s, g, segs, seglen, summ);
-    else hipLaunchKernelGGL((scan_run_kernel<OP, false, 1>), dim3((unsigned)blocks), dim3(kBlock), 0, s, g, segs, seglen, summ);
+    NT2H_SCAN_RUN(1);
+    hipLaunchKernelGGL((scan_prefix_kernel<OP, T>), dim3((unsigned)nout), dim3(kBlock), 0, s, segs, summ);
     NT2H_HIP(hipGetLastError());
-    hipLaunchKernelGGL((scan_prefix_kernel<OP>), dim3((unsigned)nout), dim3(kBlock), 0, s, segs, summ);
-    NT2H_HIP(hipGetLastError());
-    if (vec) hipLaunchKernelGGL((scan_run_kernel<OP, true, 3>), dim3((unsigned)blocks), dim3(kBlock), 0, s, g, segs, seglen, summ);
-    else hipLaunchKernelGGL((scan_run_kernel<OP, false, 3>), dim3((unsigned)blocks), dim3(kBlock), 0, s, g, segs, seglen, summ);
-    NT2H_HIP(hipGetLastError());
+    NT2H_SCAN_RUN(3);
+#undef NT2H_SCAN_RUN
     return NT2H_OK;
   }
-  const bool col4 = (lo % 4 == 0) && aligned16(x, y);
-  const int64_t units = col4 ? nout / 4 : nout;  // lanes per segment
+  const bool colv = (lo % W == 0) && aligned16(x, y);
+  const int64_t units = colv ? nout / W : nout;  // lanes per segment
   int64_t segs = ((int64_t)1 << 18) / units;
   int64_t maxsegs = (mid + 63) / 64;
   if (segs > maxsegs) segs = maxsegs;
@@ -427,9 +452,9 @@ int scan_launch(const float* x, float* y, int64_t lo, int64_t mid, int64_t hi, n
   SSumm* summ = nullptr;
 #define NT2H_SCAN_COL(PASS)                                                                                      \
   do {                                                                                                            \
-    if (col4) hipLaunchKernelGGL((scan_col4_kernel<OP, PASS>), dim3((unsigned)blocks), dim3(kBlock), 0, s, g,   \
+    if (colv) hipLaunchKernelGGL((scan_colv_kernel<OP, T, PASS>), dim3((unsigned)blocks), dim3(kBlock), 0, s, g, \
                                  segs, seglen, summ);                                                             \
-    else hipLaunchKernelGGL((scan_col_kernel<OP, PASS>), dim3((unsigned)blocks), dim3(kBlock), 0, s, g, segs,  \
+    else hipLaunchKernelGGL((scan_col_kernel<OP, T, PASS>), dim3((unsigned)blocks), dim3(kBlock), 0, s, g, segs, \
                             seglen, summ);                                                                        \
     NT2H_HIP(hipGetLastError());                                                                                  \
   } while (0)
@@ -439,12 +464,27 @@ int scan_launch(const float* x, float* y, int64_t lo, int64_t mid, int64_t hi, n
   }
   NT2H_TRY(get_scratch(st, (size_t)(nout * segs) * sizeof(SSumm), (void**)&summ));
   NT2H_SCAN_COL(1);
-  hipLaunchKernelGGL((scan_col_prefix_kernel<OP>), dim3((unsigned)((nout + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
-                     nout, segs, summ);
+  hipLaunchKernelGGL((scan_col_prefix_kernel<OP, T>), dim3((unsigned)((nout + kBlock - 1) / kBlock)), dim3(kBlock),
+                     0, s, nout, segs, summ);
   NT2H_HIP(hipGetLastError());
   NT2H_SCAN_COL(3);
 #undef NT2H_SCAN_COL
   return NT2H_OK;
+}
+
+template <class T> int scan_entry(int op, const T* x, int64_t lo, int64_t mid, int64_t hi, T* y, nt2h_stream_t s) {
+  if (op < NT2H_SCAN_CUMSUM || op > NT2H_SCAN_CUMMIN) return NT2H_ERR_BAD_OP;
+  if (lo < 0 || mid < 0 || hi < 0) return NT2H_ERR_INVALID_ARG;
+  if (lo > 0 && hi > 0 && (mid > INT64_MAX / lo || lo * mid > INT64_MAX / hi)) return NT2H_ERR_TOO_LARGE;
+  if (lo * mid * hi == 0) return NT2H_OK;
+  if (!x || !y) return NT2H_ERR_NULL_POINTER;
+  switch (op) {
+    case NT2H_SCAN_CUMSUM: return scan_launch<NT2H_SCAN_CUMSUM, T>(x, y, lo, mid, hi, s);
+    case NT2H_SCAN_CUMPROD: return scan_launch<NT2H_SCAN_CUMPROD, T>(x, y, lo, mid, hi, s);
+    case NT2H_SCAN_CUMMAX: return scan_launch<NT2H_SCAN_CUMMAX, T>(x, y, lo, mid, hi, s);
+    case NT2H_SCAN_CUMMIN: return scan_launch<NT2H_SCAN_CUMMIN, T>(x, y, lo, mid, hi, s);
+  }
+  return NT2H_ERR_BAD_OP;
 }
 
 }  // namespace nt2h
@@ -454,18 +494,10 @@ using namespace nt2h;
 extern "C" {
 
 int nt2h_scan_f32(int op, const float* x, int64_t lo, int64_t mid, int64_t hi, float* y, nt2h_stream_t s) {
-  if (op < NT2H_SCAN_CUMSUM || op > NT2H_SCAN_CUMMIN) return NT2H_ERR_BAD_OP;
-  if (lo < 0 || mid < 0 || hi < 0) return NT2H_ERR_INVALID_ARG;
-  if (lo > 0 && hi > 0 && (mid > INT64_MAX / lo || lo * mid > INT64_MAX / hi)) return NT2H_ERR_TOO_LARGE;
-  if (lo * mid * hi == 0) return NT2H_OK;
-  if (!x || !y) return NT2H_ERR_NULL_POINTER;
-  switch (op) {
-    case NT2H_SCAN_CUMSUM: return scan_launch<NT2H_SCAN_CUMSUM>(x, y, lo, mid, hi, s);
-    case NT2H_SCAN_CUMPROD: return scan_launch<NT2H_SCAN_CUMPROD>(x, y, lo, mid, hi, s);
-    case NT2H_SCAN_CUMMAX: return scan_launch<NT2H_SCAN_CUMMAX>(x, y, lo, mid, hi, s);
-    case NT2H_SCAN_CUMMIN: return scan_launch<NT2H_SCAN_CUMMIN>(x, y, lo, mid, hi, s);
-  }
-  return NT2H_ERR_BAD_OP;
+  return scan_entry<float>(op, x, lo, mid, hi, y, s);
+}
+int nt2h_scan_f64(int op, const double* x, int64_t lo, int64_t mid, int64_t hi, double* y, nt2h_stream_t s) {
+  return scan_entry<double>(op, x, lo, mid, hi, y, s);
 }
 
 }  // extern "C"

@@ -186,13 +186,25 @@ def first_non_singleton(extents) -> int:
     return 1
 
 
+def _fdt(x, what):
+    """float32 / float64 table -> (dtype, ABI suffix)."""
+    torch = _torch()
+    if x.dtype == torch.float32:
+        return torch.float32, "f32"
+    if x.dtype == torch.float64:
+        return torch.float64, "f64"
+    raise TypeError(f"{what}: float32 or float64 table, got {x.dtype}")
+
+
 def fold(op: str, x, dim: int | None = None, out=None):
     """NT2 reduction functor ``op`` (sum, prod, maximum, minimum, asum1, asum2,
     norm2, mean) of table ``x`` along 1-based NT2 dimension ``dim`` (default:
-    the first non-singleton one).  ``x`` is a contiguous float32 CUDA tensor
-    whose reversed shape is the NT2 extent; the result has extent 1 along dim."""
+    the first non-singleton one).  ``x`` is a contiguous float32/float64 CUDA
+    tensor whose reversed shape is the NT2 extent; the result has extent 1
+    along dim."""
     torch = _torch()
-    _chk(x, torch.float32, "x")
+    dt, sfx = _fdt(x, "fold")
+    _chk(x, dt, "x")
     if op not in FOLD_OPS:
         raise ValueError(f"fold: unknown functor {op!r}")
     ext = nt2_extents(x)
@@ -202,19 +214,20 @@ def fold(op: str, x, dim: int | None = None, out=None):
     oext = list(ext) + [1] * max(0, dim - len(ext))
     oext[dim - 1] = 1
     shape = tuple(reversed(oext))
-    y = torch.empty(shape, dtype=torch.float32, device=x.device) if out is None else _chk(out, torch.float32, "out")
+    y = torch.empty(shape, dtype=dt, device=x.device) if out is None else _chk(out, dt, "out")
     if y.numel() != lo * hi:
         raise ValueError(f"fold: out must hold {lo * hi} values")
-    call("nt2h_fold_f32", FOLD_OPS[op], _ptr(x), lo, mid, hi, _ptr(y), _stream())
+    call("nt2h_fold_" + sfx, FOLD_OPS[op], _ptr(x), lo, mid, hi, _ptr(y), _stream())
     return y
 
 
 def global_fold(op: str, x, out=None):
-    """globalX(x): the functor over every element (a 1-element float32 tensor)."""
+    """globalX(x): the functor over every element (a 1-element tensor of x's type)."""
     torch = _torch()
-    _chk(x, torch.float32, "x")
-    y = torch.empty(1, dtype=torch.float32, device=x.device) if out is None else _chk(out, torch.float32, "out")
-    call("nt2h_fold_f32", FOLD_OPS[op], _ptr(x), 1, x.numel(), 1, _ptr(y), _stream())
+    dt, sfx = _fdt(x, "global_fold")
+    _chk(x, dt, "x")
+    y = torch.empty(1, dtype=dt, device=x.device) if out is None else _chk(out, dt, "out")
+    call("nt2h_fold_" + sfx, FOLD_OPS[op], _ptr(x), 1, x.numel(), 1, _ptr(y), _stream())
     return y
 
 
@@ -222,16 +235,17 @@ def scan(op: str, x, dim: int | None = None, out=None):
     """NT2 scan ``op`` (cumsum, cumprod, cummax, cummin) of table ``x`` along
     1-based NT2 dimension ``dim`` (default: first non-singleton)."""
     torch = _torch()
-    _chk(x, torch.float32, "x")
+    dt, sfx = _fdt(x, "scan")
+    _chk(x, dt, "x")
     if op not in SCAN_OPS:
         raise ValueError(f"scan: unknown functor {op!r}")
     ext = nt2_extents(x)
     if dim is None:
         dim = first_non_singleton(ext)
     lo, mid, hi = lmh(ext, dim)
-    y = torch.empty_like(x) if out is None else _chk(out, torch.float32, "out")
+    y = torch.empty_like(x) if out is None else _chk(out, dt, "out")
     _same_size("scan", x, y)
-    call("nt2h_scan_f32", SCAN_OPS[op], _ptr(x), lo, mid, hi, _ptr(y), _stream())
+    call("nt2h_scan_" + sfx, SCAN_OPS[op], _ptr(x), lo, mid, hi, _ptr(y), _stream())
     return y
 
 

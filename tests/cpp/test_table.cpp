@@ -220,6 +220,16 @@ int main() {
     ok = ok && same_bits(E.to_host(), wn);
     report("scans_unit_and_bit_exact", ok);
   }
+  {  // double tables: maximum along 2 and cumsum along 1 of i + 10 j
+    std::vector<double> y(15);
+    for (int j = 1; j <= 3; ++j)
+      for (int i = 1; i <= 5; ++i) y[(i - 1) + 5 * (j - 1)] = (double)(i + 10 * j);
+    table<double> Y(y, 5, 3);
+    bool ok = nt2hip::maximum(Y, 2).to_host() == std::vector<double>{31, 32, 33, 34, 35};
+    auto c = nt2hip::cumsum(Y).to_host();
+    ok = ok && c[4] == 11 + 12 + 13 + 14 + 15 && nt2hip::globalmin(Y) == 11.0;
+    report("double_folds_scans", ok);
+  }
   std::printf("%d failure(s)\n", failures);
   return failures ? 1 : 0;
 }

@@ -27,12 +27,16 @@ def _dev_tensor(a, dev):
     return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
 
 
+def _sfx(x):
+    return "f64" if np.asarray(x).dtype == np.float64 else "f32"
+
+
 def _fold(op, x, lo, mid, hi, dev):
     import torch
 
     gx = _dev_tensor(x, dev)
-    out = torch.empty(lo * hi, dtype=torch.float32, device=dev)
-    nt2_amd.call("nt2h_fold_f32", nt2_amd.FOLD_OPS[op], gx.data_ptr(), lo, mid, hi, out.data_ptr(),
+    out = torch.empty(lo * hi, dtype=gx.dtype, device=dev)
+    nt2_amd.call("nt2h_fold_" + _sfx(x), nt2_amd.FOLD_OPS[op], gx.data_ptr(), lo, mid, hi, out.data_ptr(),
                  torch.cuda.current_stream().cuda_stream)
     return out.cpu().numpy()
 
@@ -42,7 +46,7 @@ def _scan(op, x, lo, mid, hi, dev):
 
     gx = _dev_tensor(x, dev)
     y = torch.empty_like(gx)
-    nt2_amd.call("nt2h_scan_f32", nt2_amd.SCAN_OPS[op], gx.data_ptr(), lo, mid, hi, y.data_ptr(),
+    nt2_amd.call("nt2h_scan_" + _sfx(x), nt2_amd.SCAN_OPS[op], gx.data_ptr(), lo, mid, hi, y.data_ptr(),
                  torch.cuda.current_stream().cuda_stream)
     return y.cpu().numpy()
 
@@ -58,7 +62,8 @@ def _specials_data(n, seed):
 
 
 def _bits_eq(a, b):
-    return (a.view(np.uint32) == b.view(np.uint32)) | (np.isnan(a) & np.isnan(b))
+    u = np.uint64 if a.dtype == np.float64 else np.uint32
+    return (a.view(u) == b.view(u)) | (np.isnan(a) & np.isnan(b))
 
 
 @pytest.mark.parametrize("op", ["maximum", "minimum"])
@@ -241,6 +246,62 @@ def test_scan_unit_vectors(dev, ref):
     assert _scan("cumsum", a, 3, 5, 2, dev)[-1] == 120
     p = np.full(6, 2.0, np.float32)
     assert np.array_equal(_scan("cumprod", p, 1, 6, 1, dev), 2.0 ** np.arange(1, 7))
+
+
+# ------------------------------------------------------------ double tables
+F64_SHAPES = [(1, 5000, 37), (1, (1 << 19) + 3, 1), (1, 1 << 17, 2), (1, 100, 1000), (6, 3000, 5), (1024, 300, 2),
+              (3, 5, 2)]
+
+
+@pytest.mark.parametrize("shape", F64_SHAPES)
+def test_f64_extreme_folds_and_scans_bit_identical(dev, ref, shape):
+    lo, mid, hi = shape
+    x = _specials_data(lo * mid * hi, 21).astype(np.float64)
+    x[::97] = np.float64(1.7976931348623157e308)  # DBL_MAX, beyond any float
+    for op in ("maximum", "minimum"):
+        got, want = _fold(op, x, lo, mid, hi, dev), ref.fold_lmh(op, x, lo, mid, hi)
+        assert got.dtype == np.float64 and _bits_eq(got, want).all(), op
+    for op in ("cummax", "cummin"):
+        got, want = _scan(op, x, lo, mid, hi, dev), ref.scan_lmh(op, x, lo, mid, hi)
+        assert _bits_eq(got, want).all(), op
+    inf = np.full(lo * mid * hi, -np.inf)
+    assert np.all(_fold("maximum", inf, lo, mid, hi, dev) == -np.finfo(np.float64).max)  # Maxinit = -DBL_MAX
+
+
+@pytest.mark.parametrize("shape", F64_SHAPES)
+def test_f64_sums_prod_scans(dev, ref, shape):
+    lo, mid, hi = shape
+    rng = np.random.default_rng(22)
+    x = rng.standard_normal(lo * mid * hi) + 0.25
+    exact = np.array([float(np.sum(np.array(r, dtype=np.longdouble))) for r in
+                      x.reshape(hi, mid, lo).transpose(0, 2, 1).reshape(-1, mid)]) if mid * lo * hi < 3e6 else None
+    got = _fold("sum", x, lo, mid, hi, dev)
+    mag = np.abs(x).reshape(hi, mid, lo).sum(axis=1).reshape(-1)
+    if exact is not None:  # compensated: within 1 ULP of the (extended-precision) exact total
+        assert np.all(np.abs(got - exact) <= np.spacing(np.abs(exact)) + 1e-30 * mag)
+    seq = ref.fold_lmh("sum", x, lo, mid, hi)  # NT2's sequential double sum, inside its error bound
+    assert np.all(np.abs(seq - got) <= mid * np.finfo(np.float64).eps * mag)
+    nm = _fold("norm2", x, lo, mid, hi, dev)
+    assert np.allclose(nm, np.sqrt((x * x).reshape(hi, mid, lo).sum(axis=1).reshape(-1)), rtol=1e-15)
+    p = rng.uniform(0.99, 1.01, lo * mid * hi)
+    gp = _fold("prod", p, lo, mid, hi, dev)
+    assert np.allclose(gp, np.prod(p.reshape(hi, mid, lo), axis=1).reshape(-1), rtol=mid * 2e-16 + 1e-15)
+    cs = _scan("cumsum", x, lo, mid, hi, dev)
+    wc = np.cumsum(x.reshape(hi, mid, lo), axis=1).reshape(-1)
+    cm = np.cumsum(np.abs(x).reshape(hi, mid, lo), axis=1).reshape(-1)
+    assert np.all(np.abs(cs - wc) <= mid * np.finfo(np.float64).eps * cm)  # within numpy's own error
+    cp = _scan("cumprod", p, lo, mid, hi, dev)
+    assert np.allclose(cp, np.cumprod(p.reshape(hi, mid, lo), axis=1).reshape(-1), rtol=mid * 2e-16 + 1e-15)
+
+
+def test_f64_wrappers_and_unit_tables(dev):
+    import torch
+
+    y = torch.tensor([[i + 10 * j for i in range(1, 6)] for j in range(1, 4)], dtype=torch.float64, device=dev)
+    assert ops.fold("maximum", y, 1).cpu().numpy().ravel().tolist() == [15, 25, 35]
+    assert ops.global_fold("prod", torch.arange(1, 5, dtype=torch.float64, device=dev)).item() == 24
+    a = torch.arange(1, 31, dtype=torch.float64, device=dev)
+    assert ops.scan("cumsum", a.reshape(10, 3)).cpu().numpy().ravel()[:9].tolist() == [1, 3, 6, 4, 9, 15, 7, 15, 24]
 
 
 def test_scan_in_place(dev, ref):
