@@ -22,7 +22,8 @@
 //  * combine kernels: when a run is split into segments, one workgroup per
 //    output combines the segment summaries (each lane a contiguous range of
 //    segments, then a fixed tree: the same order on every call); for the
-//    column paths one lane per output combines them in segment order.
+//    column paths one lane per output combines them in segment order, or,
+//    with few outputs and many segments, one workgroup per output.
 // Arithmetic per functor family:
 //  * sum, asum1, asum2, norm2, mean: TwoSum-compensated lanes in the table's
 //    type (asum2 adds the exact error of each square), compensated FP64
@@ -468,6 +469,31 @@ __global__ __launch_bounds__(kBlock) void fold_col_combine_kernel(Geo<T> g, int6
   else out[o] = F::finish(a, g.mid);
 }
 
+// The same with one workgroup per output, for few outputs and many segments:
+// lane l folds a contiguous range of segments in order, then a fixed tree.
+template <int OP, class T>
+__global__ __launch_bounds__(kBlock) void fold_col_combine_block_kernel(Geo<T> g, int64_t segs, T* out,
+                                                                        const Summ* summ) {
+  using F = typename Family<OP, T>::type;
+  __shared__ Summ sh[kBlock];
+  const int64_t nout = g.lo * g.hi;
+  const int64_t o = blockIdx.x;
+  const int64_t per = (segs + kBlock - 1) / kBlock;
+  const int64_t s0 = threadIdx.x * per;
+  const int64_t s1 = (s0 + per < segs) ? s0 + per : segs;
+  Summ a = F::sident();
+  for (int64_t s = s0; s < s1; ++s) a = F::scomb(a, summ[s * nout + o]);
+  a = block_tree(a, sh, [](Summ p, Summ q) { return F::scomb(p, q); });
+  T r;
+  if constexpr (F::kRescan) {
+    __shared__ typename F::L shl[kBlock];
+    r = ext_finish_block<F, T>(g, o, F::unsumm(a), shl);
+  } else {
+    r = F::finish(a, g.mid);
+  }
+  if (threadIdx.x == 0) out[o] = r;
+}
+
 // Output for an empty middle extent: NT2 folds nothing from the neutral
 // (mean: rec(0) * 0 = NaN).
 template <class T> __global__ void fold_fill_kernel(T* out, int64_t n, T v) {
@@ -527,9 +553,14 @@ int fold_launch(const T* x, int64_t lo, int64_t mid, int64_t hi, T* out, nt2h_st
   else hipLaunchKernelGGL((fold_col_kernel<OP, T>), dim3((unsigned)blocks), dim3(kBlock), 0, s, g, segs, seglen, out, summ);
   NT2H_HIP(hipGetLastError());
   if (segs > 1) {
-    const int64_t cblocks = (nout + kBlock - 1) / kBlock;
-    hipLaunchKernelGGL((fold_col_combine_kernel<OP, T>), dim3((unsigned)cblocks), dim3(kBlock), 0, s, g, segs, out,
-                       (const Summ*)summ);
+    if (nout >= 8192 || segs <= 16) {  // enough outputs for one lane each
+      const int64_t cblocks = (nout + kBlock - 1) / kBlock;
+      hipLaunchKernelGGL((fold_col_combine_kernel<OP, T>), dim3((unsigned)cblocks), dim3(kBlock), 0, s, g, segs,
+                         out, (const Summ*)summ);
+    } else {
+      hipLaunchKernelGGL((fold_col_combine_block_kernel<OP, T>), dim3((unsigned)nout), dim3(kBlock), 0, s, g, segs,
+                         out, (const Summ*)summ);
+    }
     NT2H_HIP(hipGetLastError());
   }
   return NT2H_OK;

@@ -365,19 +365,23 @@ __global__ __launch_bounds__(kBlock) void scan_col_prefix_kernel(int64_t nout, i
 }
 
 // Pass 2: per output (one workgroup), replace the segment totals by their
-// exclusive prefixes, in segment order.
+// exclusive prefixes, in segment order.  The totals of output o sit at
+// o * ostride + s * sstride (run paths: o * segs + s; column paths with few
+// outputs: s * nout + o).
 template <int OP, class T>
-__global__ __launch_bounds__(kBlock) void scan_prefix_kernel(int64_t segs, SSumm* summ) {
+__global__ __launch_bounds__(kBlock) void scan_prefix_kernel(int64_t segs, SSumm* summ, int64_t ostride,
+                                                             int64_t sstride) {
   using M = typename SFam<OP, T>::type;
   using C = typename M::C;
   __shared__ SSumm sh[kBlock];
   const int64_t o = blockIdx.x;
-  SSumm* row = summ + o * segs;
+  SSumm* base = summ + o * ostride;
+  auto row = [&](int64_t s) -> SSumm& { return base[s * sstride]; };
   const int64_t per = (segs + kBlock - 1) / kBlock;
   const int64_t s0 = threadIdx.x * per;
   const int64_t s1 = (s0 + per < segs) ? s0 + per : segs;
   C a = M::cident();
-  for (int64_t s = s0; s < s1; ++s) a = M::ccomb(a, M::unpack(row[s]));
+  for (int64_t s = s0; s < s1; ++s) a = M::ccomb(a, M::unpack(row(s)));
   sh[threadIdx.x] = M::pack(a);
   __syncthreads();
   // exclusive prefix of the lane totals, sequentially in lane order (one lane)
@@ -392,8 +396,8 @@ __global__ __launch_bounds__(kBlock) void scan_prefix_kernel(int64_t segs, SSumm
   __syncthreads();
   C pre = M::unpack(sh[threadIdx.x]);
   for (int64_t s = s0; s < s1; ++s) {
-    C t = M::unpack(row[s]);
-    row[s] = M::pack(pre);
+    C t = M::unpack(row(s));
+    row(s) = M::pack(pre);
     pre = M::ccomb(pre, t);
   }
 }
@@ -431,7 +435,8 @@ int scan_launch(const T* x, T* y, int64_t lo, int64_t mid, int64_t hi, nt2h_stre
     }
     NT2H_TRY(get_scratch(st, (size_t)blocks * sizeof(SSumm), (void**)&summ));
     NT2H_SCAN_RUN(1);
-    hipLaunchKernelGGL((scan_prefix_kernel<OP, T>), dim3((unsigned)nout), dim3(kBlock), 0, s, segs, summ);
+    hipLaunchKernelGGL((scan_prefix_kernel<OP, T>), dim3((unsigned)nout), dim3(kBlock), 0, s, segs, summ, segs,
+                       (int64_t)1);
     NT2H_HIP(hipGetLastError());
     NT2H_SCAN_RUN(3);
 #undef NT2H_SCAN_RUN
@@ -464,8 +469,12 @@ int scan_launch(const T* x, T* y, int64_t lo, int64_t mid, int64_t hi, nt2h_stre
   }
   NT2H_TRY(get_scratch(st, (size_t)(nout * segs) * sizeof(SSumm), (void**)&summ));
   NT2H_SCAN_COL(1);
-  hipLaunchKernelGGL((scan_col_prefix_kernel<OP, T>), dim3((unsigned)((nout + kBlock - 1) / kBlock)), dim3(kBlock),
-                     0, s, nout, segs, summ);
+  if (nout >= 8192 || segs <= 16)  // enough outputs for one lane each
+    hipLaunchKernelGGL((scan_col_prefix_kernel<OP, T>), dim3((unsigned)((nout + kBlock - 1) / kBlock)), dim3(kBlock),
+                       0, s, nout, segs, summ);
+  else
+    hipLaunchKernelGGL((scan_prefix_kernel<OP, T>), dim3((unsigned)nout), dim3(kBlock), 0, s, segs, summ, (int64_t)1,
+                       nout);
   NT2H_HIP(hipGetLastError());
   NT2H_SCAN_COL(3);
 #undef NT2H_SCAN_COL
