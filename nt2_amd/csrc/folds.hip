@@ -428,6 +428,22 @@ __global__ __launch_bounds__(kBlock) void fold_colv_kernel(Geo<T> g, int64_t seg
   }
 }
 
+// Fold the summaries base[s * st], s = s0..s1-1, into a in order, loading
+// eight at a time so that their latencies overlap.
+template <class F>
+__device__ __forceinline__ Summ scomb_range(Summ a, const Summ* base, int64_t s0, int64_t s1, int64_t st) {
+  int64_t s = s0;
+  for (; s + 8 <= s1; s += 8) {
+    Summ b[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) b[u] = base[(s + u) * st];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) a = F::scomb(a, b[u]);
+  }
+  for (; s < s1; ++s) a = F::scomb(a, base[s * st]);
+  return a;
+}
+
 // Combine kernel: one workgroup per output; lane l takes a contiguous range
 // of segments in order, the lanes then combine in a fixed tree (the sum and
 // product families are deterministic but not sequential in rounding; the
@@ -441,7 +457,7 @@ __global__ __launch_bounds__(kBlock) void fold_combine_kernel(Geo<T> g, int64_t 
   const int64_t s0 = threadIdx.x * per;
   const int64_t s1 = (s0 + per < segs) ? s0 + per : segs;
   Summ a = F::sident();
-  for (int64_t s = s0; s < s1; ++s) a = F::scomb(a, summ[o * segs + s]);
+  a = scomb_range<F>(a, summ + o * segs, s0, s1, 1);
   a = block_tree(a, sh, [](Summ p, Summ q) { return F::scomb(p, q); });
   T r;
   if constexpr (F::kRescan) {
@@ -464,7 +480,7 @@ __global__ __launch_bounds__(kBlock) void fold_col_combine_kernel(Geo<T> g, int6
   const int64_t o = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (o >= nout) return;
   Summ a = summ[o];
-  for (int64_t s = 1; s < segs; ++s) a = F::scomb(a, summ[s * nout + o]);
+  a = scomb_range<F>(a, summ + o, 1, segs, nout);
   if constexpr (F::kRescan) out[o] = ext_finish_thread<F, T>(g, o, F::unsumm(a));
   else out[o] = F::finish(a, g.mid);
 }
@@ -482,7 +498,7 @@ __global__ __launch_bounds__(kBlock) void fold_col_combine_block_kernel(Geo<T> g
   const int64_t s0 = threadIdx.x * per;
   const int64_t s1 = (s0 + per < segs) ? s0 + per : segs;
   Summ a = F::sident();
-  for (int64_t s = s0; s < s1; ++s) a = F::scomb(a, summ[s * nout + o]);
+  a = scomb_range<F>(a, summ + o, s0, s1, nout);
   a = block_tree(a, sh, [](Summ p, Summ q) { return F::scomb(p, q); });
   T r;
   if constexpr (F::kRescan) {

@@ -348,20 +348,53 @@ __global__ __launch_bounds__(kBlock) void scan_colv_kernel(SGeo<T> g, int64_t se
   }
 }
 
+// Replace base[s * st], s = s0..s1-1, by their exclusive prefixes starting
+// from pre, in order (eight loads in flight); returns the running total.
+template <class M>
+__device__ __forceinline__ typename M::C exclusive_range(typename M::C pre, SSumm* base, int64_t s0, int64_t s1,
+                                                         int64_t st) {
+  int64_t s = s0;
+  for (; s + 8 <= s1; s += 8) {
+    SSumm b[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) b[u] = base[(s + u) * st];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      base[(s + u) * st] = M::pack(pre);
+      pre = M::ccomb(pre, M::unpack(b[u]));
+    }
+  }
+  for (; s < s1; ++s) {
+    typename M::C t = M::unpack(base[s * st]);
+    base[s * st] = M::pack(pre);
+    pre = M::ccomb(pre, t);
+  }
+  return pre;
+}
+// Total of base[s * st], s = s0..s1-1, in order (eight loads in flight).
+template <class M>
+__device__ __forceinline__ typename M::C total_range(const SSumm* base, int64_t s0, int64_t s1, int64_t st) {
+  typename M::C a = M::cident();
+  int64_t s = s0;
+  for (; s + 8 <= s1; s += 8) {
+    SSumm b[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) b[u] = base[(s + u) * st];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) a = M::ccomb(a, M::unpack(b[u]));
+  }
+  for (; s < s1; ++s) a = M::ccomb(a, M::unpack(base[s * st]));
+  return a;
+}
+
 // Pass 2 of the column paths: one lane per output replaces its segment
 // totals (segment-major) by their exclusive prefixes, in segment order.
 template <int OP, class T>
 __global__ __launch_bounds__(kBlock) void scan_col_prefix_kernel(int64_t nout, int64_t segs, SSumm* summ) {
   using M = typename SFam<OP, T>::type;
-  using C = typename M::C;
   const int64_t o = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (o >= nout) return;
-  C run = M::cident();
-  for (int64_t s = 0; s < segs; ++s) {
-    C t = M::unpack(summ[s * nout + o]);
-    summ[s * nout + o] = M::pack(run);
-    run = M::ccomb(run, t);
-  }
+  exclusive_range<M>(M::cident(), summ + o, 0, segs, nout);
 }
 
 // Pass 2: per output (one workgroup), replace the segment totals by their
@@ -376,30 +409,22 @@ __global__ __launch_bounds__(kBlock) void scan_prefix_kernel(int64_t segs, SSumm
   __shared__ SSumm sh[kBlock];
   const int64_t o = blockIdx.x;
   SSumm* base = summ + o * ostride;
-  auto row = [&](int64_t s) -> SSumm& { return base[s * sstride]; };
   const int64_t per = (segs + kBlock - 1) / kBlock;
   const int64_t s0 = threadIdx.x * per;
   const int64_t s1 = (s0 + per < segs) ? s0 + per : segs;
-  C a = M::cident();
-  for (int64_t s = s0; s < s1; ++s) a = M::ccomb(a, M::unpack(row(s)));
+  C a = total_range<M>(base, s0, s1, sstride);
+  // inclusive prefix of the lane totals in lane order (Hillis-Steele: each
+  // step combines a lower range with the adjacent higher one), then shifted
   sh[threadIdx.x] = M::pack(a);
   __syncthreads();
-  // exclusive prefix of the lane totals, sequentially in lane order (one lane)
-  if (threadIdx.x == 0) {
-    C run = M::cident();
-    for (int l = 0; l < kBlock; ++l) {
-      C t = M::unpack(sh[l]);
-      sh[l] = M::pack(run);
-      run = M::ccomb(run, t);
-    }
+  for (int d = 1; d < kBlock; d <<= 1) {
+    C lower = ((int)threadIdx.x >= d) ? M::unpack(sh[threadIdx.x - d]) : M::cident();
+    __syncthreads();
+    if ((int)threadIdx.x >= d) sh[threadIdx.x] = M::pack(M::ccomb(lower, M::unpack(sh[threadIdx.x])));
+    __syncthreads();
   }
-  __syncthreads();
-  C pre = M::unpack(sh[threadIdx.x]);
-  for (int64_t s = s0; s < s1; ++s) {
-    C t = M::unpack(row(s));
-    row(s) = M::pack(pre);
-    pre = M::ccomb(pre, t);
-  }
+  C pre = threadIdx.x == 0 ? M::cident() : M::unpack(sh[threadIdx.x - 1]);
+  exclusive_range<M>(pre, base, s0, s1, sstride);
 }
 
 template <int OP, class T>
