@@ -47,23 +47,31 @@ def _traffic_from_profile():
     return None
 
 
-def _cpu_baseline(inputs_np, budget_s=12.0):
-    """The oracle's float fwd+adj under NT2's OpenMP grain policy, host cores, bounded sample."""
+def _cpu_baseline(inputs_np, budget_s=10.0):
+    """The oracle's NT2 functors restated branch-free and vectorised (AVX2,
+    #pragma omp simd; bit-identical to the scalar restatement) under NT2's
+    OpenMP grain policy, on the host cores, over a bounded sample: median of
+    the timed passes after one warm-up.  The scalar restatement is timed
+    briefly beside it for reference."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import nt2ref
 
     lib = nt2ref.lib()
-    n = 1 << 22  # sample: the first 2^22 options of the workload
-    ins = [np.ascontiguousarray(a[:n]) for a in inputs_np]
-    outs = [np.empty(n, np.float32) for _ in range(6)]
-    nt2ref.cpu_bs_fwd_adj_f32(ins, outs)  # warm-up, first touch
-    reps, t0 = 0, time.perf_counter()
-    while True:
-        nt2ref.cpu_bs_fwd_adj_f32(ins, outs)
-        reps += 1
-        el = time.perf_counter() - t0
-        if el >= budget_s or reps >= 2000:
-            break
+
+    def median_rate(fn, n, budget, max_reps):
+        ins = [np.ascontiguousarray(a[:n]) for a in inputs_np]
+        outs = [np.empty(n, np.float32) for _ in range(6)]
+        fn(ins, outs)  # warm-up, first touch
+        ts, t_all = [], time.perf_counter()
+        while (time.perf_counter() - t_all < budget or len(ts) < 5) and len(ts) < max_reps:
+            t0 = time.perf_counter()
+            fn(ins, outs)
+            ts.append(time.perf_counter() - t0)
+        return n / float(np.median(ts)), len(ts), time.perf_counter() - t_all
+
+    n = len(inputs_np[0])  # the whole workload: 805 MB of streams, past any cache
+    rate, reps, el = median_rate(nt2ref.cpu_simd_bs_fwd_adj_f32, n, budget_s, 2000)
+    scalar_rate, _, _ = median_rate(nt2ref.cpu_bs_fwd_adj_f32, 1 << 22, 2.0, 200)
     model = ""
     try:
         with open("/proc/cpuinfo") as f:
@@ -71,14 +79,15 @@ def _cpu_baseline(inputs_np, budget_s=12.0):
     except OSError:
         pass
     return {
-        "value": n * reps / el,
+        "value": rate,
         "unit": "options/s",
         "cores": int(lib.nt2ref_max_threads()),
         "kind": "port",
-        "sample": f"first 2^22 options of the workload, {reps} full passes in {el:.1f} s "
-                  f"(NT2 scalar functors restated, OpenMP static grains of L2/4 floats)",
-        "gbps": BYTES_PER_OPTION * n * reps / el / 1e9,
+        "sample": f"the whole 2^24-option workload; median of {reps} passes in {el:.1f} s "
+                  f"(NT2 functors restated branch-free, AVX2 via omp simd, OpenMP static grains of L2/4 floats)",
+        "gbps": BYTES_PER_OPTION * rate / 1e9,
         "cpu_model": model,
+        "scalar_port_options_per_s": scalar_rate,  # first 2^22 options, median over ~2 s
     }
 
 

@@ -68,6 +68,8 @@ def lib() -> C.CDLL:
         L.nt2ref_nt2_ulpdist_f64.restype = C.c_double
         L.nt2ref_exhaustive_hist_f32.argtypes = [C.c_int, C.c_float, C.c_float, P]
         L.nt2ref_max_threads.restype = C.c_int
+        L.nt2ref_cpu_simd_bs_fwd_adj_f32.argtypes = [P] * 12 + [i64, i64]
+        L.nt2ref_cpu_simd_map_f32.argtypes = [C.c_int, P, P, i64]
         for nm in ("nt2ref_fold_lmh_f32", "nt2ref_fold_lmh_f64", "nt2ref_scan_lmh_f32", "nt2ref_scan_lmh_f64"):
             getattr(L, nm).argtypes = [C.c_int, P, i64, i64, i64, P]
         _lib = L
@@ -182,6 +184,19 @@ def bs_fwd_adj(S, K, T, r, sig, w, dtype=np.float32):
 def cpu_bs_fwd_adj_f32(ins, outs, l2_bytes=1 << 20):
     """Multithreaded CPU baseline (NT2 shared-memory grain policy); arrays preallocated."""
     lib().nt2ref_cpu_bs_fwd_adj_f32(*[_p(a) for a in ins], *[_p(a) for a in outs], ins[0].size, l2_bytes)
+
+
+def cpu_simd_bs_fwd_adj_f32(ins, outs, l2_bytes=1 << 20):
+    """The vectorised (branch-free, AVX2) CPU baseline under the same grain policy."""
+    lib().nt2ref_cpu_simd_bs_fwd_adj_f32(*[_p(a) for a in ins], *[_p(a) for a in outs], ins[0].size, l2_bytes)
+
+
+def cpu_simd_map_f32(op: str, x) -> np.ndarray:
+    """Branch-free float exp, log, erfc, normcdf, normpdf of the vectorised baseline."""
+    x = _f32(x)
+    y = np.empty_like(x)
+    lib().nt2ref_cpu_simd_map_f32(OPS[op], _p(x), _p(y), x.size)
+    return y
 
 
 def colfold_f32(X, s, k, p) -> np.ndarray:
