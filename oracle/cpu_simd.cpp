@@ -18,6 +18,10 @@
 #include <algorithm>
 #include <cmath>
 
+// One clone per instruction set, chosen at load time by the host CPU: AVX-512
+// (16 lanes), AVX2 (8 lanes), or the x86-64 baseline.
+#define NT2REF_CLONES __attribute__((target_clones("avx512f", "avx2", "default")))
+
 namespace {
 
 inline float fb(uint32_t u) {
@@ -131,7 +135,7 @@ inline float normpdf_bf(float a0) { return exp_bf(-0.5f * (a0 * a0)) * fb(0x3ecc
 
 // The option loop of one grain: the oracle's bs_fwd_adj<float> statement by
 // statement.
-void bs_grain(const float* __restrict S, const float* __restrict K, const float* __restrict Tm,
+NT2REF_CLONES void bs_grain(const float* __restrict S, const float* __restrict K, const float* __restrict Tm,
               const float* __restrict r, const float* __restrict sig, const float* __restrict w,
               float* __restrict price, float* __restrict dS, float* __restrict dK, float* __restrict dT,
               float* __restrict dr, float* __restrict dsig, int64_t n) {
@@ -164,7 +168,7 @@ extern "C" {
 // The branch-free functors over an array (op ids as the oracle's: 0 exp,
 // 1 log, 4 erfc, 5 normcdf, 9 normpdf), for checking them against the
 // scalar restatement.
-void nt2ref_cpu_simd_map_f32(int op, const float* x, float* y, int64_t n) {
+NT2REF_CLONES void nt2ref_cpu_simd_map_f32(int op, const float* x, float* y, int64_t n) {
 #pragma omp simd
   for (int64_t i = 0; i < n; ++i) {
     float v = x[i];
