@@ -14,10 +14,11 @@
 //  * run kernel (lo == 1, mid >= 512): one workgroup per (run, segment); the
 //    lanes stride over the segment (16-byte loads when aligned), and the
 //    workgroup combines its lanes in a fixed LDS tree;
-//  * column kernel (lo > 1, or short runs): one lane per (output, segment)
+//  * column kernels (lo > 1, or short runs): one lane per (output, segment)
 //    folding sequentially — or, when lo is a multiple of the 16-byte width
-//    W (4 floats, 2 doubles), per group of W adjacent outputs with one
-//    16-byte load per step, four steps in flight;
+//    W (4 floats, 2 doubles), the tile kernel: lanes own groups of W
+//    adjacent outputs with one 16-byte load per row, and the row phases of a
+//    workgroup share a segment and combine in LDS;
 //    neighbouring lanes read neighbouring addresses;
 //  * combine kernels: when a run is split into segments, one workgroup per
 //    output combines the segment summaries (each lane a contiguous range of
@@ -378,52 +379,76 @@ __global__ __launch_bounds__(kBlock) void fold_col_kernel(Geo<T> g, int64_t segs
 }
 
 // Column kernel, 16-byte form (lo % W == 0, x 16-byte aligned; W = 4 floats
-// or 2 doubles): one lane per (group of W adjacent outputs, segment); each
-// step along m loads one 16-byte vector holding element m of the W outputs,
-// four steps at a time so that several loads are in flight per lane.
+// or 2 doubles).  A workgroup is R = kBlock / P row phases of P lanes (P a
+// power of two <= 64): lane (r, p) folds the W adjacent outputs of group
+// q = qb * P + p — one 16-byte vector per row — over rows m0 + r, m0 + r + R,
+// ... of its segment, four loads in flight.  So one wave reads 1 KiB of
+// consecutive addresses per load (64 groups of one row, or 64 / P adjacent
+// rows when the rows are short), and the R phases of the workgroup walk down
+// the segment side by side.  Their R partial summaries per output are then
+// combined in LDS by a fixed tree, so that a segment needs one summary per
+// output however many lanes folded it.
 template <int OP, class T>
-__global__ __launch_bounds__(kBlock) void fold_colv_kernel(Geo<T> g, int64_t segs, int64_t seglen, T* out,
-                                                           Summ* summ) {
+__global__ __launch_bounds__(kBlock) void fold_tile_kernel(Geo<T> g, int lgP, int64_t qblocks, int64_t segs,
+                                                           int64_t seglen, T* out, Summ* summ) {
   using F = typename Family<OP, T>::type;
   using L = typename F::L;
   using V = typename Vec16<T>::type;
   constexpr int W = Vec16<T>::n;
+  const int P = 1 << lgP, R = kBlock >> lgP;
+  const int p = threadIdx.x & (P - 1), r = threadIdx.x >> lgP;
   const int64_t nq = g.lo * g.hi / W;
-  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (t >= nq * segs) return;
-  const int64_t s = t / nq, q = t - s * nq;
+  const int64_t s = blockIdx.x / qblocks, q = (blockIdx.x - s * qblocks) * P + p;
+  const bool live = q < nq;
   const int64_t o = W * q;  // first of the W outputs; the group never straddles h since lo % W == 0
   const int64_t h = o / g.lo, i = o - h * g.lo;
-  const V* p = reinterpret_cast<const V*>(g.x + i + g.lo * g.mid * h);
+  const V* base = reinterpret_cast<const V*>(g.x + i + g.lo * g.mid * h);
   const int64_t step = g.lo / W;  // vectors between m and m + 1
   const int64_t m0 = s * seglen;
-  const int64_t m1 = (m0 + seglen < g.mid) ? m0 + seglen : g.mid;
+  const int64_t m1 = !live ? m0 : (m0 + seglen < g.mid) ? m0 + seglen : g.mid;
   L a[W];
 #pragma unroll
   for (int k = 0; k < W; ++k) a[k] = F::ident();
-  int64_t m = m0;
-  for (; m + 4 <= m1; m += 4) {
+  int64_t m = m0 + r;
+  for (; m + 3 * R < m1; m += 4 * R) {
     V v[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) v[u] = __builtin_nontemporal_load(p + (m + u) * step);
+    for (int u = 0; u < 4; ++u) v[u] = __builtin_nontemporal_load(base + (m + u * R) * step);
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
 #pragma unroll
-      for (int k = 0; k < W; ++k) F::add(a[k], v[u][k], m + u);
+      for (int k = 0; k < W; ++k) F::add(a[k], v[u][k], m + u * R);
     }
   }
-  for (; m < m1; ++m) {
-    V v = __builtin_nontemporal_load(p + m * step);
+  for (; m < m1; m += R) {
+    V v = __builtin_nontemporal_load(base + m * step);
 #pragma unroll
     for (int k = 0; k < W; ++k) F::add(a[k], v[k], m);
   }
+  Summ acc[W];
+#pragma unroll
+  for (int k = 0; k < W; ++k) acc[k] = F::summ(a[k]);
+  __shared__ Summ sh[kBlock / 2 * W];
+  for (int half = R >> 1; half >= 1; half >>= 1) {
+    if (r >= half && r < 2 * half) {
+#pragma unroll
+      for (int k = 0; k < W; ++k) sh[((r - half) * P + p) * W + k] = acc[k];
+    }
+    __syncthreads();
+    if (r < half) {
+#pragma unroll
+      for (int k = 0; k < W; ++k) acc[k] = F::scomb(acc[k], sh[(r * P + p) * W + k]);
+    }
+    __syncthreads();
+  }
+  if (r != 0 || !live) return;
 #pragma unroll
   for (int k = 0; k < W; ++k) {
     if (segs > 1) {
-      summ[s * (g.lo * g.hi) + o + k] = F::summ(a[k]);
+      summ[s * (g.lo * g.hi) + o + k] = acc[k];
     } else {
-      if constexpr (F::kRescan) out[o + k] = ext_finish_thread<F, T>(g, o + k, a[k]);
-      else out[o + k] = F::finish(F::summ(a[k]), g.mid);
+      if constexpr (F::kRescan) out[o + k] = ext_finish_thread<F, T>(g, o + k, F::unsumm(acc[k]));
+      else out[o + k] = F::finish(acc[k], g.mid);
     }
   }
 }
@@ -547,25 +572,44 @@ int fold_launch(const T* x, int64_t lo, int64_t mid, int64_t hi, T* out, nt2h_st
     }
     return NT2H_OK;
   }
-  // column kernel: aim for >= 2^18 lanes; segments of at least 16 elements
   const bool colv = (lo % W == 0) && aligned16(x);
-  const int64_t units = colv ? nout / W : nout;  // lanes per segment
-  int64_t segs = ((int64_t)1 << 18) / units;
-  int64_t maxsegs = (mid + 15) / 16;
-  if (segs > maxsegs) segs = maxsegs;
-  if (segs > 4096) segs = 4096;
-  if (segs < 1) segs = 1;
-  int64_t seglen = (mid + segs - 1) / segs;
-  segs = (mid + seglen - 1) / seglen;
+  int64_t segs, seglen, blocks;
+  int lgP = 0;
+  int64_t qblocks = 0;
+  if (colv) {
+    // tile kernel: about 2^17 lanes with four 16-byte loads each in flight
+    // (8 MiB) — measured to stream strided rows fastest, ahead of twice or
+    // half that — and each lane at least four rows of its segment
+    const int64_t nq = nout / W;
+    while (lgP < 6 && ((int64_t)1 << lgP) < nq) ++lgP;
+    const int64_t R = kBlock >> lgP;
+    qblocks = (nq + ((int64_t)1 << lgP) - 1) >> lgP;
+    segs = ((int64_t)1 << 17) / (qblocks * kBlock);
+    const int64_t maxsegs = (mid + 4 * R - 1) / (4 * R);
+    if (segs > maxsegs) segs = maxsegs;
+    if (segs < 1) segs = 1;
+    seglen = (mid + segs - 1) / segs;
+    segs = (mid + seglen - 1) / seglen;
+    blocks = qblocks * segs;
+  } else {
+    // scalar column kernel: aim for >= 2^18 lanes; segments of at least 16
+    // elements
+    segs = ((int64_t)1 << 18) / nout;
+    int64_t maxsegs = (mid + 15) / 16;
+    if (segs > maxsegs) segs = maxsegs;
+    if (segs > 4096) segs = 4096;
+    if (segs < 1) segs = 1;
+    seglen = (mid + segs - 1) / segs;
+    segs = (mid + seglen - 1) / seglen;
+    blocks = (nout * segs + kBlock - 1) / kBlock;
+  }
   Summ* summ = nullptr;
   if (segs > 1) {
     if (nout > 0x7fffffffLL) return NT2H_ERR_TOO_LARGE;
     NT2H_TRY(get_scratch(st, (size_t)(nout * segs) * sizeof(Summ), (void**)&summ));
   }
-  const int64_t lanes = units * segs;
-  const int64_t blocks = (lanes + kBlock - 1) / kBlock;
   if (blocks > 0x7fffffffLL) return NT2H_ERR_TOO_LARGE;
-  if (colv) hipLaunchKernelGGL((fold_colv_kernel<OP, T>), dim3((unsigned)blocks), dim3(kBlock), 0, s, g, segs, seglen, out, summ);
+  if (colv) hipLaunchKernelGGL((fold_tile_kernel<OP, T>), dim3((unsigned)blocks), dim3(kBlock), 0, s, g, lgP, qblocks, segs, seglen, out, summ);
   else hipLaunchKernelGGL((fold_col_kernel<OP, T>), dim3((unsigned)blocks), dim3(kBlock), 0, s, g, segs, seglen, out, summ);
   NT2H_HIP(hipGetLastError());
   if (segs > 1) {

@@ -24,7 +24,8 @@
 //    walks the segment in tiles of 256 x W elements (W = 4 floats or 2
 //    doubles per 16 bytes; lane l holds elements W*l..W*l+W-1), scans each
 //    tile in the workgroup (wave shuffles, then the four wave totals
-//    through LDS) and carries the tile total forward;
+//    through LDS) and carries the tile total forward, loading the next tile
+//    while it scans the current one;
 //  * column kernel (lo > 1, or short runs): one lane per (output, segment)
 //    running the recurrence — or, when lo % W == 0, per group of W adjacent
 //    outputs with 16-byte loads and stores; neighbouring lanes touch
@@ -77,6 +78,7 @@ template <class T> struct MSum {
     two_sum_s(a.s, b.s, t, e);
     return V{t, (a.c + b.c) + e};
   }
+  static constexpr bool kComm = true;  // the total may be accumulated in any fixed order
   __device__ static V shfl_up(const V& v, int d) { return V{shfl_up_t(v.s, d), shfl_up_t(v.c, d)}; }
   __device__ static C cident() { return C{0.0, 0.0}; }
   __device__ static C ccomb(const C& a, const C& b) {
@@ -112,6 +114,7 @@ template <class T> struct MProd {
   __device__ static V ident() { return V{1.0, 0}; }
   __device__ static V lift(T x) { return norm(V{(double)x, 0}); }
   __device__ static V comb(const V& a, const V& b) { return norm(V{a.m * b.m, a.e + b.e}); }
+  static constexpr bool kComm = true;
   __device__ static V shfl_up(const V& v, int d) { return V{shfl_up_t(v.m, d), shfl_up_t(v.e, d)}; }
   __device__ static C cident() { return ident(); }
   __device__ static C cadd(const C& c, const V& v) { return comb(c, v); }
@@ -141,6 +144,7 @@ template <bool MAX, class T> struct MExt {
     const bool keep_a = MAX ? (a.acc > b.acc) : (a.acc < b.acc);
     return V{a.nan, keep_a ? a.acc : b.acc};
   }
+  static constexpr bool kComm = false;  // a NaN resets: only ordered combines
   __device__ static V shfl_up(const V& v, int d) { return V{shfl_up_t(v.nan, d), shfl_up_t(v.acc, d)}; }
   __device__ static C cident() { return ident(); }
   __device__ static C cadd(const C& c, const V& v) { return comb(c, v); }
@@ -156,73 +160,89 @@ template <class T> struct SFam<NT2H_SCAN_CUMPROD, T> { using type = MProd<T>; };
 template <class T> struct SFam<NT2H_SCAN_CUMMAX, T> { using type = MExt<true, T>; };
 template <class T> struct SFam<NT2H_SCAN_CUMMIN, T> { using type = MExt<false, T>; };
 
-// Ordered exclusive scan of one value per lane over the workgroup; returns
-// the exclusive prefix of this lane and the workgroup total in *total.
-// lds holds kWaves values.  All threads must call it.
-template <class M>
-__device__ __forceinline__ typename M::V block_exscan(typename M::V v, typename M::V* lds, typename M::V* total) {
-  using V = typename M::V;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  V inc = v;
+// Ordered inclusive scan of one value per lane over the 64 lanes of a wave
+// (Hillis-Steele: each step combines a lane with the lane d below it, the
+// lower lane on the left).
+template <class M> __device__ __forceinline__ typename M::V wave_inscan(typename M::V v) {
+  const int lane = threadIdx.x & 63;
 #pragma unroll
   for (int d = 1; d < 64; d <<= 1) {
-    V o = M::shfl_up(inc, d);
-    if (lane >= d) inc = M::comb(o, inc);
+    typename M::V o = M::shfl_up(v, d);
+    if (lane >= d) v = M::comb(o, v);
   }
-  if (lane == 63) lds[wave] = inc;
-  __syncthreads();
-  V wpre = M::ident();
-  V tot = M::ident();
-#pragma unroll
-  for (int w = 0; w < kWaves; ++w) {
-    V t = lds[w];
-    if (w < wave) wpre = M::comb(wpre, t);
-    tot = M::comb(tot, t);
-  }
-  __syncthreads();  // lds is reused by the next call
-  V ex = M::shfl_up(inc, 1);
-  if (lane == 0) ex = M::ident();
-  *total = tot;
-  return M::comb(wpre, ex);
+  return v;
 }
 
-// Run kernel body: scan elements [m0, m1) of run o starting from carry;
-// WRITE = false computes only the total (pass 1 of segmented scans).
+// Run kernel body, one workgroup: scan elements [m0, m1) of run o starting
+// from carry, in tiles of kBlock x W elements (lane l holds elements
+// W*l..W*l+W-1 of the tile; W = 4 floats or 2 doubles with 16-byte
+// accesses, else 1).  Each wave scans its quarter of the tile with
+// shuffles; the four wave totals pass through LDS (two slots used in turn,
+// so one barrier per tile suffices), and the tile total is carried forward.
+// The next tile is loaded before the current one is scanned, so the loads
+// stay in flight across the scan and the barrier.  WRITE = false computes
+// only the total (pass 1 of segmented scans); for monoids whose total may be
+// taken in any fixed order (sum, product) each lane then folds its own
+// elements and the lanes are combined once, in lane order, at the end.
 template <class M, class T, bool VEC, bool WRITE>
-__device__ typename M::C run_segment(const SGeo<T>& g, int64_t o, int64_t m0, int64_t m1, typename M::C carry) {
+__device__ __forceinline__ typename M::C run_segment(const SGeo<T>& g, int64_t o, int64_t m0, int64_t m1,
+                                                     typename M::C carry) {
   using V = typename M::V;
   using VT = typename Vec16<T>::type;
-  constexpr int W = Vec16<T>::n;
-  __shared__ V lds[kWaves];
+  constexpr int W = VEC ? Vec16<T>::n : 1;
+  constexpr int64_t kTile = (int64_t)kBlock * W;
+  constexpr bool kLaneTotals = !WRITE && M::kComm;
+  __shared__ V lds[2][kWaves];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const T* run = g.x + o * g.mid;
   T* out = g.y + o * g.mid;
-  if (VEC) {
-    for (int64_t base = m0; base < m1; base += W * kBlock) {
-      const int64_t m = base + W * (int64_t)threadIdx.x;
-      T e[W];
-      if (m + W - 1 < m1) {
-        VT q = __builtin_nontemporal_load(reinterpret_cast<const VT*>(run + m));
+  auto load = [&](int64_t base, T(&e)[W]) {
+    const int64_t m = base + W * (int64_t)threadIdx.x;
+    if (VEC && m + W - 1 < m1) {
+      VT q = __builtin_nontemporal_load(reinterpret_cast<const VT*>(run + m));
 #pragma unroll
-        for (int k = 0; k < W; ++k) e[k] = q[k];
-      } else {
+      for (int k = 0; k < W; ++k) e[k] = q[k];
+    } else {
 #pragma unroll
-        for (int k = 0; k < W; ++k) e[k] = (m + k < m1) ? run[m + k] : T(0);
+      for (int k = 0; k < W; ++k) e[k] = (m + k < m1) ? run[m + k] : T(0);
+    }
+  };
+  T cur[W] = {}, nxt[W] = {};
+  if (m0 < m1) load(m0, cur);
+  V lanes = M::ident();
+  int par = 0;
+  for (int64_t base = m0; base < m1; base += kTile) {
+    if (base + kTile < m1) load(base + kTile, nxt);
+    const int64_t m = base + W * (int64_t)threadIdx.x;
+    V p[W];
+    V acc = M::ident();
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+      V xk = (m + k < m1) ? M::lift(cur[k]) : M::ident();
+      acc = (k == 0) ? xk : M::comb(acc, xk);
+      p[k] = acc;
+    }
+    if (kLaneTotals) {
+      lanes = M::comb(lanes, acc);
+    } else {
+      V inc = wave_inscan<M>(acc);
+      if (lane == 63) lds[par][wave] = inc;
+      __syncthreads();
+      V wpre = M::ident(), tot = M::ident();
+#pragma unroll
+      for (int w = 0; w < kWaves; ++w) {
+        V t = lds[par][w];
+        if (w < wave) wpre = M::comb(wpre, t);
+        tot = M::comb(tot, t);
       }
-      V p[W];
-      V acc = M::ident();
-#pragma unroll
-      for (int k = 0; k < W; ++k) {
-        V xk = (m + k < m1) ? M::lift(e[k]) : M::ident();
-        acc = (k == 0) ? xk : M::comb(acc, xk);
-        p[k] = acc;
-      }
-      V tot;
-      V ex = block_exscan<M>(acc, lds, &tot);
       if (WRITE) {
+        V ex = M::shfl_up(inc, 1);
+        if (lane == 0) ex = M::ident();
+        ex = M::comb(wpre, ex);
         T r[W];
 #pragma unroll
         for (int k = 0; k < W; ++k) r[k] = M::fin(carry, M::comb(ex, p[k]));
-        if (m + W - 1 < m1) {
+        if (VEC && m + W - 1 < m1) {
           VT q;
 #pragma unroll
           for (int k = 0; k < W; ++k) q[k] = r[k];
@@ -234,16 +254,19 @@ __device__ typename M::C run_segment(const SGeo<T>& g, int64_t o, int64_t m0, in
         }
       }
       carry = M::cadd(carry, tot);
+      par ^= 1;
     }
-  } else {
-    for (int64_t base = m0; base < m1; base += kBlock) {
-      const int64_t m = base + threadIdx.x;
-      V xk = (m < m1) ? M::lift(run[m]) : M::ident();
-      V tot;
-      V ex = block_exscan<M>(xk, lds, &tot);
-      if (WRITE && m < m1) out[m] = M::fin(carry, M::comb(ex, xk));
-      carry = M::cadd(carry, tot);
-    }
+#pragma unroll
+    for (int k = 0; k < W; ++k) cur[k] = nxt[k];
+  }
+  if (kLaneTotals) {
+    V inc = wave_inscan<M>(lanes);
+    if (lane == 63) lds[par][wave] = inc;
+    __syncthreads();
+    V tot = M::ident();
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) tot = M::comb(tot, lds[par][w]);
+    carry = M::cadd(carry, tot);
   }
   return carry;
 }
@@ -434,15 +457,17 @@ int scan_launch(const T* x, T* y, int64_t lo, int64_t mid, int64_t hi, nt2h_stre
   const int64_t nout = lo * hi;
   SGeo<T> g{x, y, lo, mid, hi};
   if (lo == 1 && mid >= 512) {
-    const int64_t tile = (int64_t)W * kBlock;
-    int64_t segs = (8192 + hi - 1) / hi;
-    int64_t maxsegs = (mid + 8 * tile - 1) / (8 * tile);  // segments of at least 8 tiles
+    // one workgroup per (run, segment): about 8192 of them, segments of at
+    // least 8 tiles
+    const bool vec = (mid % W == 0) && aligned16(x, y);
+    const int64_t tile = vec ? (int64_t)W * kBlock : kBlock;
+    int64_t segs = (8192 + nout - 1) / nout;
+    int64_t maxsegs = (mid + 8 * tile - 1) / (8 * tile);
     if (segs > maxsegs) segs = maxsegs;
     if (segs < 1) segs = 1;
     int64_t seglen = (mid + segs - 1) / segs;
     seglen = (seglen + tile - 1) / tile * tile;
     segs = (mid + seglen - 1) / seglen;
-    const bool vec = (mid % W == 0) && aligned16(x, y);
     const int64_t blocks = nout * segs;
     if (blocks > 0x7fffffffLL) return NT2H_ERR_TOO_LARGE;
     SSumm* summ = nullptr;
@@ -469,7 +494,9 @@ int scan_launch(const T* x, T* y, int64_t lo, int64_t mid, int64_t hi, nt2h_stre
   }
   const bool colv = (lo % W == 0) && aligned16(x, y);
   const int64_t units = colv ? nout / W : nout;  // lanes per segment
-  int64_t segs = ((int64_t)1 << 18) / units;
+  // 16-byte lanes: about 2^17 of them, four loads each in flight (8 MiB),
+  // measured to stream strided rows fastest; scalar lanes: 2^18
+  int64_t segs = ((int64_t)1 << (colv ? 17 : 18)) / units;
   int64_t maxsegs = (mid + 63) / 64;
   if (segs > maxsegs) segs = maxsegs;
   if (segs > 4096) segs = 4096;
