@@ -28,12 +28,14 @@ constexpr int kCounterStride = 64;
 // Reduction workspace of one (device, stream): per-block partials
 // (kMaxPartials x kMaxFoldBlocks doubles), per-group partials (kMaxPartials x
 // kMaxGroups), kMaxGroups + 1 ticket counters (kCounterStride apart) that the
-// kernels reset after use, and a result slot for host-pointer outputs.
+// kernels reset after use, a result slot for host-pointer outputs, and the
+// block partials of folds that finish in a second launch.
 struct Workspace {
   double* partials;
   double* gpartials;
   unsigned* counters;
-  double* result;  // kMaxPartials doubles
+  double* result;     // kMaxPartials doubles
+  double* bpartials;  // block partials of two-launch folds (as partials); may be null
 };
 // Workgroups for a fold over `units` vectors, one vector per lane per step.
 inline unsigned fold_blocks(int64_t units) {
@@ -175,9 +177,21 @@ __device__ __forceinline__ bool grid_combine(const double (&bsum)[NV], const Wor
   if (t2 != ng - 1) return false;
 #pragma unroll
   for (int v = 0; v < NV; ++v) acc[v] = 0.0;
-  for (unsigned i = lane; i < ng; i += 64) {
+  // lane l sums group partials l, l + 64, l + 128, ... in that order; four
+  // of them are fetched at a time so that their round trips overlap
+  for (unsigned i0 = lane; i0 < ng; i0 += 256) {
+    double b[NV][4];
 #pragma unroll
-    for (int v = 0; v < NV; ++v) acc[v] += fetch(&ws.gpartials[v * ng + i]);
+    for (int j = 0; j < 4; ++j) {
+      const unsigned i = i0 + 64u * j;
+#pragma unroll
+      for (int v = 0; v < NV; ++v) b[v][j] = (i < ng) ? fetch(&ws.gpartials[v * ng + i]) : 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+#pragma unroll
+      for (int v = 0; v < NV; ++v) acc[v] += b[v][j];
+    }
   }
 #pragma unroll
   for (int m = 1; m < 64; m <<= 1) {
@@ -190,6 +204,62 @@ __device__ __forceinline__ bool grid_combine(const double (&bsum)[NV], const Wor
     __hip_atomic_store(&ws.counters[kMaxGroups * kCounterStride], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   return lane == 0;
+}
+
+// Second launch of a two-launch fold.  A single-launch fold keeps wave 0 of
+// every workgroup alive for two memory round trips (publish, then ticket),
+// which on a grid of tens of thousands of workgroups holds enough wave
+// slots to cost ~10% of the bandwidth; the first launch therefore only
+// stores each workgroup's NV partials to ws.bpartials[v * nb + b] with plain
+// stores, and this kernel (a few workgroups) sums them: workgroup k takes a
+// contiguous range of blocks, lane-strided, then a fixed shuffle tree and
+// the waves in order, and the ticket combine finishes over the workgroups.
+// The same order on every call.
+template <int NV>
+__global__ __launch_bounds__(kBlock) void fold_partials_kernel(int nb, Workspace ws, double* out) {
+  const int per = (nb + (int)gridDim.x - 1) / (int)gridDim.x;
+  const int b0 = (int)blockIdx.x * per;
+  const int b1 = (b0 + per < nb) ? b0 + per : nb;
+  double s[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) s[v] = 0.0;
+  for (int i = b0 + (int)threadIdx.x; i < b1; i += kBlock) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) s[v] += ws.bpartials[(int64_t)v * nb + i];
+  }
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) s[v] += shfl_xor_d(s[v], m);
+  }
+  __shared__ double lw[NV][kWaves];
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) lw[v][threadIdx.x >> 6] = s[v];
+  }
+  __syncthreads();
+  double bsum[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    bsum[v] = 0.0;
+    if (threadIdx.x == 0) {
+      double t = lw[v][0];
+      for (int w = 1; w < kWaves; ++w) t += lw[v][w];
+      bsum[v] = t;
+    }
+  }
+  double res[NV];
+  if (grid_combine<NV>(bsum, ws, res)) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) out[v] = res[v];
+  }
+}
+// Workgroups of the second launch for nb block partials.
+inline unsigned partials_blocks(int64_t nb) {
+  int64_t g = (nb + 1023) / 1024;
+  if (g < 1) g = 1;
+  if (g > 64) g = 64;
+  return (unsigned)g;
 }
 
 }  // namespace nt2h

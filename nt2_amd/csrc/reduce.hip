@@ -1,13 +1,14 @@
 // Whole-table folds (NT2 inner_fold_ with plus / Zero: sum, globalsum, dot)
 // as one single-pass HIP kernel each.
 //
-// Each lane accumulates a grid-strided slice with 128-bit loads (one vector
-// per lane per step, up to 64K workgroups so that the hardware keeps
-// launching waves while others wait on memory), the four waves of a
-// workgroup combine through shuffles and LDS, and a two-level ticket
-// (groups of 64 workgroups, then the groups) combines the partials in FP64
-// in index order — no second launch, no float atomics, the same bits on
-// every run for a given size.
+// Each lane accumulates a grid-strided slice with 128-bit loads (two
+// vectors per lane in flight, up to 64K workgroups so that the hardware
+// keeps launching waves while others wait on memory), the four waves of a
+// workgroup combine through shuffles and LDS, and the workgroup partials
+// are combined in FP64 in a fixed order: by a two-level ticket (groups of
+// 64 workgroups, then the groups) in the same launch for small grids, and
+// for large grids by a second, small launch (fold_partials_kernel) — no
+// float atomics, the same bits on every run for a given size.
 // NT2's counterpart: scalar fold_step (core/reduction/include/nt2/core/
 // functions/details/fold_step.hpp:31-43), SIMD inner_fold (simd/
 // inner_fold.hpp:44-67) and the OpenMP block fold merged under omp
@@ -75,7 +76,26 @@ template <> struct LaneAcc<NT2H_SUM_PAIRWISE> {
   }
 };
 
-template <int MODE, bool DOT, bool VEC>
+// One 16-byte step of a lane: four elements (DOT: four exact products
+// whose rounding errors enter the compensation).
+template <int MODE, bool DOT>
+__device__ __forceinline__ void fold_step4(LaneAcc<MODE>& acc, const f32x4& a, const f32x4& b) {
+  if (DOT) {
+    float p0 = a.x * b.x, p1 = a.y * b.y, p2 = a.z * b.z, p3 = a.w * b.w;
+    acc.add4(p0, p1, p2, p3);
+    if (MODE == NT2H_SUM_KAHAN) {
+      float e = (__builtin_fmaf(a.x, b.x, -p0) + __builtin_fmaf(a.y, b.y, -p1)) +
+                (__builtin_fmaf(a.z, b.z, -p2) + __builtin_fmaf(a.w, b.w, -p3));
+      acc.add_err(e);
+    }
+  } else {
+    acc.add4(a.x, a.y, a.z, a.w);
+  }
+}
+
+// TWO: store the block partial for fold_partials_kernel instead of taking
+// the ticket (large grids).
+template <int MODE, bool DOT, bool VEC, bool TWO>
 __global__ __launch_bounds__(kBlock) void fold_f32_kernel(const float* __restrict__ x, const float* __restrict__ y,
                                                           int64_t n, Workspace ws, double* out) {
   LaneAcc<MODE> acc;
@@ -84,20 +104,24 @@ __global__ __launch_bounds__(kBlock) void fold_f32_kernel(const float* __restric
   int64_t done = 0;
   if (VEC) {
     const int64_t nv = n / 4;
-    for (int64_t i = gid; i < nv; i += stride) {
-      f32x4 a = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(x) + i);
+    const f32x4* x4 = reinterpret_cast<const f32x4*>(x);
+    const f32x4* y4 = reinterpret_cast<const f32x4*>(y);
+    int64_t i = gid;
+    // two steps at a time, both loads issued before either is folded
+    for (; i + stride < nv; i += 2 * stride) {
+      f32x4 a0 = __builtin_nontemporal_load(x4 + i), a1 = __builtin_nontemporal_load(x4 + i + stride);
+      f32x4 b0 = a0, b1 = a1;
       if (DOT) {
-        f32x4 b = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(y) + i);
-        float p0 = a.x * b.x, p1 = a.y * b.y, p2 = a.z * b.z, p3 = a.w * b.w;
-        acc.add4(p0, p1, p2, p3);
-        if (MODE == NT2H_SUM_KAHAN) {
-          float e = (__builtin_fmaf(a.x, b.x, -p0) + __builtin_fmaf(a.y, b.y, -p1)) +
-                    (__builtin_fmaf(a.z, b.z, -p2) + __builtin_fmaf(a.w, b.w, -p3));
-          acc.add_err(e);
-        }
-      } else {
-        acc.add4(a.x, a.y, a.z, a.w);
+        b0 = __builtin_nontemporal_load(y4 + i);
+        b1 = __builtin_nontemporal_load(y4 + i + stride);
       }
+      fold_step4<MODE, DOT>(acc, a0, b0);
+      fold_step4<MODE, DOT>(acc, a1, b1);
+    }
+    for (; i < nv; i += stride) {
+      f32x4 a = __builtin_nontemporal_load(x4 + i);
+      f32x4 b = DOT ? __builtin_nontemporal_load(y4 + i) : a;
+      fold_step4<MODE, DOT>(acc, a, b);
     }
     done = nv * 4;
   }
@@ -116,6 +140,10 @@ __global__ __launch_bounds__(kBlock) void fold_f32_kernel(const float* __restric
   Comp c[1] = {acc.finish()};
   double bsum[1] = {0.0};
   block_reduce_comp<1>(c, lds, bsum);
+  if (TWO) {
+    if (threadIdx.x == 0) ws.bpartials[blockIdx.x] = bsum[0];
+    return;
+  }
   double res[1] = {0.0};
   if (grid_combine<1>(bsum, ws, res)) out[0] = res[0];
 }
@@ -146,13 +174,26 @@ __global__ __launch_bounds__(kBlock) void fold_f64_kernel(const double* __restri
   if (grid_combine<1>(bsum, ws, res)) out[0] = res[0];
 }
 
+// Grids of at least kTwoLaunchBlocks workgroups finish in a second launch
+// (fold_partials_kernel); smaller ones, where a second launch would cost
+// more than the tickets, combine in the same launch.
+constexpr unsigned kTwoLaunchBlocks = 4096;
+
 template <int MODE, bool DOT>
 static void launch_f32(bool vec, unsigned g, const float* x, const float* y, int64_t n, const Workspace& ws,
                        double* dst, hipStream_t st) {
+  if (g >= kTwoLaunchBlocks) {
+    if (vec)
+      hipLaunchKernelGGL((fold_f32_kernel<MODE, DOT, true, true>), dim3(g), dim3(kBlock), 0, st, x, y, n, ws, dst);
+    else
+      hipLaunchKernelGGL((fold_f32_kernel<MODE, DOT, false, true>), dim3(g), dim3(kBlock), 0, st, x, y, n, ws, dst);
+    hipLaunchKernelGGL((fold_partials_kernel<1>), dim3(partials_blocks(g)), dim3(kBlock), 0, st, (int)g, ws, dst);
+    return;
+  }
   if (vec)
-    hipLaunchKernelGGL((fold_f32_kernel<MODE, DOT, true>), dim3(g), dim3(kBlock), 0, st, x, y, n, ws, dst);
+    hipLaunchKernelGGL((fold_f32_kernel<MODE, DOT, true, false>), dim3(g), dim3(kBlock), 0, st, x, y, n, ws, dst);
   else
-    hipLaunchKernelGGL((fold_f32_kernel<MODE, DOT, false>), dim3(g), dim3(kBlock), 0, st, x, y, n, ws, dst);
+    hipLaunchKernelGGL((fold_f32_kernel<MODE, DOT, false, false>), dim3(g), dim3(kBlock), 0, st, x, y, n, ws, dst);
 }
 
 static int fold_f32(const float* x, const float* y, int64_t n, int mode, double* out, nt2h_stream_t s) {

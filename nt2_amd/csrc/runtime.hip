@@ -53,6 +53,7 @@ int get_workspace(nt2h_stream_t s, Workspace* ws) {
   NT2H_HIP(hipMalloc((void**)&w.gpartials, (size_t)kMaxPartials * kMaxGroups * sizeof(double)));
   NT2H_HIP(hipMalloc((void**)&w.counters, (size_t)(kMaxGroups + 1) * kCounterStride * sizeof(unsigned)));
   NT2H_HIP(hipMalloc((void**)&w.result, kMaxPartials * sizeof(double)));
+  NT2H_HIP(hipMalloc((void**)&w.bpartials, (size_t)kMaxPartials * kMaxFoldBlocks * sizeof(double)));
   // zero the tickets once, synchronously; kernels reset them after each use
   NT2H_HIP(hipMemset(w.counters, 0, (size_t)(kMaxGroups + 1) * kCounterStride * sizeof(unsigned)));
   g_ws[key] = w;
