@@ -274,13 +274,14 @@ template <class T> struct Family<F_PROD, T> { using type = FProd<T>; };
 template <class T> struct Family<F_MAX, T> { using type = FExt<true, T>; };
 template <class T> struct Family<F_MIN, T> { using type = FExt<false, T>; };
 
-// Fixed-order LDS tree over the workgroup; every thread receives the result.
-template <class S, class Comb>
+// Fixed-order LDS tree over a workgroup of NB lanes; every thread receives
+// the result.
+template <int NB = kBlock, class S, class Comb>
 __device__ __forceinline__ S block_tree(S v, S* sh, Comb comb) {
   sh[threadIdx.x] = v;
   __syncthreads();
 #pragma unroll
-  for (int s = kBlock / 2; s > 0; s >>= 1) {
+  for (int s = NB / 2; s > 0; s >>= 1) {
     if ((int)threadIdx.x < s) sh[threadIdx.x] = comb(sh[threadIdx.x], sh[threadIdx.x + s]);
     __syncthreads();
   }
@@ -292,7 +293,7 @@ __device__ __forceinline__ S block_tree(S v, S* sh, Comb comb) {
 // Finish an extreme-value fold of output o from its summary a; all threads
 // of the workgroup call this together (uniform control flow) and thread 0
 // gets the result.
-template <class F, class T>
+template <class F, class T, int NB = kBlock>
 __device__ T ext_finish_block(const Geo<T>& g, int64_t o, typename F::L a, typename F::L* sh) {
   const int64_t mid = g.mid;
   if (a.nan < 0) return F::pick(a);
@@ -300,8 +301,8 @@ __device__ T ext_finish_block(const Geo<T>& g, int64_t o, typename F::L a, typen
   if (a.j > a.nan) return a.v;
   // the extreme lies before the last NaN: fold the tail after it
   typename F::L t = F::ident();
-  for (int64_t m = a.nan + 1 + threadIdx.x; m < mid; m += kBlock) F::add(t, g.x[g.at(o, m)], m);
-  t = block_tree(t, sh, [](typename F::L p, typename F::L q) { return F::comb(p, q); });
+  for (int64_t m = a.nan + 1 + threadIdx.x; m < mid; m += NB) F::add(t, g.x[g.at(o, m)], m);
+  t = block_tree<NB>(t, sh, [](typename F::L p, typename F::L q) { return F::comb(p, q); });
   return t.v;  // the tail holds no NaN and is not empty
 }
 template <class F, class T>
@@ -469,25 +470,26 @@ __device__ __forceinline__ Summ scomb_range(Summ a, const Summ* base, int64_t s0
   return a;
 }
 
-// Combine kernel: one workgroup per output; lane l takes a contiguous range
-// of segments in order, the lanes then combine in a fixed tree (the sum and
+// Combine kernel: one workgroup of NB lanes per output (NB = 1024 for the
+// many segments of a whole-table fold); lane l takes a contiguous range of
+// segments in order, the lanes then combine in a fixed tree (the sum and
 // product families are deterministic but not sequential in rounding; the
 // extreme-value summaries are order-free).
-template <int OP, class T>
-__global__ __launch_bounds__(kBlock) void fold_combine_kernel(Geo<T> g, int64_t segs, T* out, const Summ* summ) {
+template <int OP, class T, int NB>
+__global__ __launch_bounds__(NB) void fold_combine_kernel(Geo<T> g, int64_t segs, T* out, const Summ* summ) {
   using F = typename Family<OP, T>::type;
-  __shared__ Summ sh[kBlock];
+  __shared__ Summ sh[NB];
   const int64_t o = blockIdx.x;
-  const int64_t per = (segs + kBlock - 1) / kBlock;
+  const int64_t per = (segs + NB - 1) / NB;
   const int64_t s0 = threadIdx.x * per;
   const int64_t s1 = (s0 + per < segs) ? s0 + per : segs;
   Summ a = F::sident();
   a = scomb_range<F>(a, summ + o * segs, s0, s1, 1);
-  a = block_tree(a, sh, [](Summ p, Summ q) { return F::scomb(p, q); });
+  a = block_tree<NB>(a, sh, [](Summ p, Summ q) { return F::scomb(p, q); });
   T r;
   if constexpr (F::kRescan) {
-    __shared__ typename F::L shl[kBlock];
-    r = ext_finish_block<F, T>(g, o, F::unsumm(a), shl);
+    __shared__ typename F::L shl[NB];
+    r = ext_finish_block<F, T, NB>(g, o, F::unsumm(a), shl);
   } else {
     r = F::finish(a, g.mid);
   }
@@ -566,8 +568,12 @@ int fold_launch(const T* x, int64_t lo, int64_t mid, int64_t hi, T* out, nt2h_st
     else hipLaunchKernelGGL((fold_run_kernel<OP, T, false>), dim3((unsigned)blocks), dim3(kBlock), 0, s, g, segs, seglen, out, summ);
     NT2H_HIP(hipGetLastError());
     if (segs > 1) {
-      hipLaunchKernelGGL((fold_combine_kernel<OP, T>), dim3((unsigned)nout), dim3(kBlock), 0, s, g, segs, out,
-                         (const Summ*)summ);
+      if (segs >= 4096)
+        hipLaunchKernelGGL((fold_combine_kernel<OP, T, 1024>), dim3((unsigned)nout), dim3(1024), 0, s, g, segs,
+                           out, (const Summ*)summ);
+      else
+        hipLaunchKernelGGL((fold_combine_kernel<OP, T, kBlock>), dim3((unsigned)nout), dim3(kBlock), 0, s, g,
+                           segs, out, (const Summ*)summ);
       NT2H_HIP(hipGetLastError());
     }
     return NT2H_OK;
