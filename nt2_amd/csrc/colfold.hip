@@ -83,12 +83,21 @@ __global__ __launch_bounds__(kBlock) void colfold_kernel(const float* __restrict
   ColAcc<GRAD> acc;
   int64_t done = 0;
   if (VEC) {
+    // the next vector is loaded before the current one is evaluated, so
+    // that its latency hides behind the arithmetic
     const int64_t nv = rows / 4;
     const f32x4* c4 = reinterpret_cast<const f32x4*>(col);
-    for (int64_t i = threadIdx.x; i < nv; i += kBlock) {
+    // (unconditionally: the last iteration reloads the last vector, unused)
+    int64_t i = threadIdx.x;
+    if (i < nv) {
       f32x4 v = __builtin_nontemporal_load(c4 + i);
-      float x[4] = {v.x, v.y, v.z, v.w};
-      acc.add4(x, m2, kj, pj);
+      for (; i < nv; i += kBlock) {
+        const int64_t inext = (i + kBlock < nv) ? i + kBlock : nv - 1;
+        f32x4 nx = __builtin_nontemporal_load(c4 + inext);
+        float x[4] = {v.x, v.y, v.z, v.w};
+        acc.add4(x, m2, kj, pj);
+        v = nx;
+      }
     }
     done = nv * 4;
   }
