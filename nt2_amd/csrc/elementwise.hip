@@ -76,17 +76,24 @@ __global__ __launch_bounds__(kBlock) void ew_kernel(Streams<T, NI, NO> st, int64
   }
 }
 
+// Grid: one 16-byte vector per lane (measured 8-20% faster at 2^28
+// elements than 65536 grid-striding workgroups for the light functors and
+// saxpy); the erfc family, whose long bodies favour fewer, longer-lived
+// workgroups, passes max_blocks = 65536.
+constexpr int64_t kEwOnePerLane = int64_t(1) << 30;
+
 template <class T, int NI, int NO, class F>
-static int launch_ew(const Streams<T, NI, NO>& st, int64_t n, F f, nt2h_stream_t s) {
+static int launch_ew(const Streams<T, NI, NO>& st, int64_t n, F f, nt2h_stream_t s,
+                     int64_t max_blocks = kEwOnePerLane) {
   if (n == 0) return NT2H_OK;
   bool al = true;
   for (int k = 0; k < NI; ++k) al = al && aligned16(st.in[k]);
   for (int k = 0; k < NO; ++k) al = al && aligned16(st.out[k]);
   if (al) {
-    unsigned g = stream_grid(n, Vec16<T>::n);
+    unsigned g = stream_grid(n, Vec16<T>::n, max_blocks);
     hipLaunchKernelGGL((ew_kernel<T, NI, NO, true, F>), dim3(g), dim3(kBlock), 0, as_hip(s), st, n, f);
   } else {
-    unsigned g = stream_grid(n, 1);
+    unsigned g = stream_grid(n, 1, max_blocks);
     hipLaunchKernelGGL((ew_kernel<T, NI, NO, false, F>), dim3(g), dim3(kBlock), 0, as_hip(s), st, n, f);
   }
   NT2H_HIP(hipGetLastError());
@@ -145,9 +152,9 @@ static int map_dispatch(int op, const T* x, T* y, int64_t n, nt2h_stream_t s) {
     case NT2H_OP_EXP: return launch_ew(st, n, OpExp<P>{}, s);
     case NT2H_OP_LOG: return launch_ew(st, n, OpLog<P>{}, s);
     case NT2H_OP_SQRT: return launch_ew(st, n, OpSqrt<P>{}, s);
-    case NT2H_OP_ERF: return launch_ew(st, n, OpErf<P>{}, s);
-    case NT2H_OP_ERFC: return launch_ew(st, n, OpErfc<P>{}, s);
-    case NT2H_OP_NORMCDF: return launch_ew(st, n, OpNormcdf<P>{}, s);
+    case NT2H_OP_ERF: return launch_ew(st, n, OpErf<P>{}, s, 65536);
+    case NT2H_OP_ERFC: return launch_ew(st, n, OpErfc<P>{}, s, 65536);
+    case NT2H_OP_NORMCDF: return launch_ew(st, n, OpNormcdf<P>{}, s, 65536);
     case NT2H_OP_TANH: return launch_ew(st, n, OpTanh<P>{}, s);
     case NT2H_OP_SIN: return launch_ew(st, n, OpSin<P>{}, s);
     case NT2H_OP_COS: return launch_ew(st, n, OpCos<P>{}, s);
