@@ -601,7 +601,7 @@ template <class P> NT2H_FD int reduce_d(double x, double* xr) {
 // fdlibm-medium in float (<= 2^6 pi), the same medium in double after
 // conversion (<= 2^18 pi); beyond that NT2 calls fdlibm's float Payne-Hanek,
 // replaced here by the integer Payne-Hanek above (correctly rounded xr).
-NT2H_FD int reduce_exact_f(float x, float* xr) {
+NT2H_FD int reduce_exact_f_near(float x, float* xr) {
   if (!(x > 0x1.921fb6p-1f)) { *xr = x; return 0; }
   if (!(x > 0x1.921fb6p+0f)) {
     float t = x - 0x1.921f00p+0f;
@@ -633,17 +633,23 @@ NT2H_FD int reduce_exact_f(float x, float* xr) {
     *xr = r - w;
     return ((int32_t)fn) & 3;
   }
-  if (!(x > 0x1.921fb6p+19f)) {  // 2^18 pi (float): convert to double and run the double ladder
-    double xrd;
-    int n = reduce_d<Exact>((double)x, &xrd);
-    *xr = (float)xrd;
-    return n;
-  }
+  // up to 2^18 pi (float): convert to double and run the double ladder
+  double xrd;
+  int n = reduce_d<Exact>((double)x, &xrd);
+  *xr = (float)xrd;
+  return n;
+}
+// Beyond 2^18 pi: the integer Payne-Hanek.
+constexpr float kExactFarF = 0x1.921fb6p+19f;
+NT2H_FD int reduce_exact_f_far(float x, float* xr) {
   uint32_t b = __float_as_uint(x);
   double rh, rl;
   int q = payne_hanek((uint64_t)((b & 0x7fffffu) | 0x800000u), (int)(b >> 23) - 150, &rh, &rl);
   *xr = (float)rh;
   return q;
+}
+NT2H_FD int reduce_exact_f(float x, float* xr) {
+  return (x > kExactFarF) ? reduce_exact_f_far(x, xr) : reduce_exact_f_near(x, xr);
 }
 
 // f_trig_evaluation.hpp:40-48
@@ -658,26 +664,35 @@ template <class P> NT2H_FD float sin_eval_f(float z, float x) {
 template <class P> NT2H_FD int reduce_f(float x, float* xr) {
   return P::fused ? reduce_fused_f(x, xr) : reduce_exact_f(x, xr);
 }
-// trig_base.hpp:131-161
-template <class P> NT2H_FD float cos_f(float a0) {
-  float x = __builtin_fabsf(a0), xr;
-  bool bad = !(x <= 0x1.fffffep+127f);
-  int n = reduce_f<P>(bad ? 0.0f : x, &xr);
+// trig_base.hpp:131-161: cos / sin of a0 from the quadrant n and reduced
+// argument xr of |a0| (inf and NaN give NaN)
+template <class P> NT2H_FD float cos_from(float a0, int n, float xr) {
+  bool bad = !(__builtin_fabsf(a0) <= 0x1.fffffep+127f);
   float z = xr * xr;
   float r = (n & 1) ? sin_eval_f<P>(z, xr) : cos_eval_f<P>(z);
   uint32_t sign = (uint32_t)(((n & 1) ^ ((n & 2) >> 1))) << 31;
   r = __uint_as_float(__float_as_uint(r) ^ sign);
   return bad ? __builtin_nanf("") : r;
 }
-template <class P> NT2H_FD float sin_f(float a0) {
-  float x = __builtin_fabsf(a0), xr;
-  bool bad = !(x <= 0x1.fffffep+127f);
-  int n = reduce_f<P>(bad ? 0.0f : x, &xr);
+template <class P> NT2H_FD float sin_from(float a0, int n, float xr) {
+  bool bad = !(__builtin_fabsf(a0) <= 0x1.fffffep+127f);
   float z = xr * xr;
   float r = (n & 1) ? cos_eval_f<P>(z) : sin_eval_f<P>(z, xr);
   uint32_t sign = (__float_as_uint(a0) & 0x80000000u) ^ ((uint32_t)(n & 2) << 30);
   r = __uint_as_float(__float_as_uint(r) ^ sign);
   return bad ? __builtin_nanf("") : r;
+}
+template <class P> NT2H_FD float cos_f(float a0) {
+  float x = __builtin_fabsf(a0), xr;
+  bool bad = !(x <= 0x1.fffffep+127f);
+  int n = reduce_f<P>(bad ? 0.0f : x, &xr);
+  return cos_from<P>(a0, n, xr);
+}
+template <class P> NT2H_FD float sin_f(float a0) {
+  float x = __builtin_fabsf(a0), xr;
+  bool bad = !(x <= 0x1.fffffep+127f);
+  int n = reduce_f<P>(bad ? 0.0f : x, &xr);
+  return sin_from<P>(a0, n, xr);
 }
 // sin and cos of one argument sharing the reduction (Fused kernels).
 template <class P> NT2H_FD void sincos_f(float a0, float* s, float* c) {

@@ -145,8 +145,83 @@ template <class P> struct OpExp10 {
   __device__ __forceinline__ void operator()(const float (&in)[1], float (&out)[1]) const { out[0] = exp10_f<P>(in[0]); }
 };
 
+// EXACT float sin / cos over 16-byte vectors with the Payne-Hanek
+// arguments (|x| > 2^18 pi) compacted.  The integer Payne-Hanek costs more
+// than the rest of the evaluation together, and run in place it occupies a
+// whole wave for each of the lane's four elements whenever any lane needs
+// it.  Instead each wave evaluates the ladder of NT2's reduction for its
+// other arguments, gathers the far ones (ballot + prefix count) into an LDS
+// list, reduces that list with all 64 lanes, and reads the results back; the
+// polynomial evaluation then runs on every element.  The arithmetic per
+// element is unchanged, so the results are those of sin_f / cos_f.
+template <bool COS>
+__global__ __launch_bounds__(kBlock) void trig_exact_kernel(const float* __restrict__ x, float* __restrict__ y,
+                                                            int64_t n) {
+  __shared__ float phx[kWaves][4 * 64];
+  __shared__ float phr[kWaves][4 * 64];
+  __shared__ int phq[kWaves][4 * 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));  // lanes under this one
+  const int64_t nv = n / 4;
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  // every thread of the workgroup runs the same iterations (barriers inside)
+  for (int64_t base = (int64_t)blockIdx.x * kBlock; base < nv; base += stride) {
+    const int64_t i = base + threadIdx.x;
+    const bool act = i < nv;
+    f32x4 v = act ? __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(x) + i) : f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    float xr[4];
+    int q[4], slot[4];
+    bool far[4];
+    int total = 0;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const float ax = __builtin_fabsf(v[e]);
+      const bool bad = !(ax <= 0x1.fffffep+127f);
+      far[e] = !bad && ax > kExactFarF;
+      xr[e] = 0.0f;
+      q[e] = 0;
+      if (!far[e]) q[e] = reduce_exact_f_near(bad ? 0.0f : ax, &xr[e]);
+      const uint64_t m = __ballot(far[e]);
+      slot[e] = total + __popcll(m & below);
+      if (far[e]) phx[w][slot[e]] = ax;
+      total += __popcll(m);
+    }
+    __syncthreads();
+    for (int j = lane; j < total; j += 64) {
+      float r;
+      phq[w][j] = reduce_exact_f_far(phx[w][j], &r);
+      phr[w][j] = r;
+    }
+    __syncthreads();
+    f32x4 o;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      if (far[e]) {
+        xr[e] = phr[w][slot[e]];
+        q[e] = phq[w][slot[e]];
+      }
+      o[e] = COS ? cos_from<Exact>(v[e], q[e], xr[e]) : sin_from<Exact>(v[e], q[e], xr[e]);
+    }
+    if (act) __builtin_nontemporal_store(o, reinterpret_cast<f32x4*>(y) + i);
+    __syncthreads();  // the lists are rewritten by the next iteration
+  }
+  const int64_t t = 4 * nv + (int64_t)blockIdx.x * kBlock + threadIdx.x;  // tail [4 nv, n)
+  if (t < n) y[t] = COS ? cos_f<Exact>(x[t]) : sin_f<Exact>(x[t]);
+}
+
 template <class T, class P>
 static int map_dispatch(int op, const T* x, T* y, int64_t n, nt2h_stream_t s) {
+  if constexpr (sizeof(T) == 4 && !P::fused) {
+    if ((op == NT2H_OP_SIN || op == NT2H_OP_COS) && n > 0 && aligned16(x) && aligned16(y)) {
+      const unsigned g = stream_grid(n, 4, 65536);
+      if (op == NT2H_OP_SIN)
+        hipLaunchKernelGGL((trig_exact_kernel<false>), dim3(g), dim3(kBlock), 0, as_hip(s), x, y, n);
+      else
+        hipLaunchKernelGGL((trig_exact_kernel<true>), dim3(g), dim3(kBlock), 0, as_hip(s), x, y, n);
+      NT2H_HIP(hipGetLastError());
+      return NT2H_OK;
+    }
+  }
   Streams<T, 1, 1> st{{x}, {y}};
   switch (op) {
     case NT2H_OP_EXP: return launch_ew(st, n, OpExp<P>{}, s);

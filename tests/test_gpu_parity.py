@@ -115,6 +115,31 @@ def test_map_f32_exact_bit_identical(dev, ref, fn):
         assert eq.all(), (x[~eq][:8], got[~eq][:8], want[~eq][:8])
 
 
+@pytest.mark.parametrize("fn", ["sin", "cos"])
+@pytest.mark.parametrize("lo,hi", [(-1e6, 1e6), (8.0e5, 8.4e5), (-3e7, 3e7), (-60, 60)])
+def test_trig_exact_compacted_equals_scalar_path(dev, fn, lo, hi):
+    """EXACT float sin/cos over aligned tables gathers the Payne-Hanek
+    arguments of each wave into one list; the scalar path (an unaligned
+    table) evaluates every element in place.  Same arithmetic: bit-identical,
+    tails and specials included; mixes of near and far arguments per wave
+    from none to all."""
+    import torch
+
+    rng = np.random.default_rng(77)
+    n = (1 << 18) + 3  # a tail of three elements after the 16-byte vectors
+    x = rng.uniform(lo, hi, n).astype(np.float32)
+    x[: len(SPECIALS)] = np.array(SPECIALS, dtype=np.float32)
+    aligned = _gpu(x, dev)
+    assert aligned.data_ptr() % 16 == 0
+    padded = _gpu(np.concatenate([[0], x]).astype(np.float32), dev)[1:]
+    assert padded.data_ptr() % 16 != 0
+    got = ops.map(fn, aligned, nt2_amd.EXACT).cpu().numpy()
+    want = ops.map(fn, padded, nt2_amd.EXACT).cpu().numpy()
+    torch.cuda.synchronize()
+    eq = _bits_equal(got, want)
+    assert eq.all(), (x[~eq][:8], got[~eq][:8], want[~eq][:8])
+
+
 @pytest.mark.parametrize("fn", F32_OPS)
 def test_map_f32_fused_bound(dev, ref, fn):
     x = _args(fn, 1 << 20)
