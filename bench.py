@@ -48,8 +48,9 @@ def _traffic_from_profile():
 
 
 def _cpu_baseline(inputs_np, budget_s=10.0):
-    """The oracle's NT2 functors restated branch-free and vectorised (AVX2,
-    #pragma omp simd; bit-identical to the scalar restatement) under NT2's
+    """The oracle's NT2 functors restated branch-free and vectorised
+    (#pragma omp simd in AVX-512 / AVX2 / baseline clones picked by the host
+    CPU; bit-identical to the scalar restatement) under NT2's
     OpenMP grain policy, on the host cores, over a bounded sample: median of
     the timed passes after one warm-up.  The scalar restatement is timed
     briefly beside it for reference."""
@@ -84,9 +85,10 @@ def _cpu_baseline(inputs_np, budget_s=10.0):
         "cores": int(lib.nt2ref_max_threads()),
         "kind": "port",
         "sample": f"the whole 2^24-option workload; median of {reps} passes in {el:.1f} s "
-                  f"(NT2 functors restated branch-free, AVX2 via omp simd, OpenMP static grains of L2/4 floats)",
+                  f"(NT2 functors restated branch-free, vectorised with omp simd in AVX-512, AVX2 and baseline clones picked by the host CPU, OpenMP static grains of L2/4 floats)",
         "gbps": BYTES_PER_OPTION * rate / 1e9,
         "cpu_model": model,
+        "isa": nt2ref.cpu_simd_isa(),
         "scalar_port_options_per_s": scalar_rate,  # first 2^22 options, median over ~2 s
     }
 

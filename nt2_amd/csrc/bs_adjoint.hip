@@ -183,7 +183,7 @@ __global__ __launch_bounds__(kBlock) void bs_fwd_adj_f32_kernel(BSAdjArgs a, int
 }
 
 template <bool TOTALS>
-__global__ __launch_bounds__(kBlock) void bs_fwd_adj_f64_kernel(BSAdjArgsD a, int64_t n, Workspace ws,
+__global__ __launch_bounds__(kBlock, 4) void bs_fwd_adj_f64_kernel(BSAdjArgsD a, int64_t n, Workspace ws,
                                                                 double* totals) {
   double acc[2] = {0.0, 0.0};
   const int64_t stride = (int64_t)gridDim.x * kBlock;

@@ -165,6 +165,15 @@ NT2REF_CLONES void bs_grain(const float* __restrict S, const float* __restrict K
 
 extern "C" {
 
+// The instruction set the clones above run with on this host (the loader
+// picks the first supported of avx512f, avx2, baseline).
+const char* nt2ref_cpu_simd_isa() {
+  __builtin_cpu_init();
+  if (__builtin_cpu_supports("avx512f")) return "avx512f";
+  if (__builtin_cpu_supports("avx2")) return "avx2";
+  return "x86-64";
+}
+
 // The branch-free functors over an array (op ids as the oracle's: 0 exp,
 // 1 log, 4 erfc, 5 normcdf, 9 normpdf), for checking them against the
 // scalar restatement.

@@ -70,6 +70,8 @@ def lib() -> C.CDLL:
         L.nt2ref_max_threads.restype = C.c_int
         L.nt2ref_cpu_simd_bs_fwd_adj_f32.argtypes = [P] * 12 + [i64, i64]
         L.nt2ref_cpu_simd_map_f32.argtypes = [C.c_int, P, P, i64]
+        L.nt2ref_cpu_simd_isa.restype = C.c_char_p
+        L.nt2ref_cpu_simd_isa.argtypes = []
         for nm in ("nt2ref_fold_lmh_f32", "nt2ref_fold_lmh_f64", "nt2ref_scan_lmh_f32", "nt2ref_scan_lmh_f64"):
             getattr(L, nm).argtypes = [C.c_int, P, i64, i64, i64, P]
         _lib = L
@@ -189,6 +191,11 @@ def cpu_bs_fwd_adj_f32(ins, outs, l2_bytes=1 << 20):
 def cpu_simd_bs_fwd_adj_f32(ins, outs, l2_bytes=1 << 20):
     """The vectorised (branch-free, AVX2) CPU baseline under the same grain policy."""
     lib().nt2ref_cpu_simd_bs_fwd_adj_f32(*[_p(a) for a in ins], *[_p(a) for a in outs], ins[0].size, l2_bytes)
+
+
+def cpu_simd_isa() -> str:
+    """Instruction set of the vectorised CPU clones on this host."""
+    return lib().nt2ref_cpu_simd_isa().decode()
 
 
 def cpu_simd_map_f32(op: str, x) -> np.ndarray:
