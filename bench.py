@@ -285,7 +285,8 @@ def report_placement(dev, stream, n=1 << 24):
 def report_folds(dev, stream):
     """The other reduction functors and the scans: whole-table over 2^28
     floats, along each dimension of a 16384 x 8192 table, along dimension 2
-    of a 64 x 2^21 table (few outputs) and of a 256 x 4096 x 256 table
+    of a 64 x 2^21 and a 4 x 2^26 table (few outputs) and of a 256 x 4096 x
+    256 table
     (decimal GB/s of algorithmic traffic: 4 B/element for a fold, 8
     B/element for a scan)."""
     import torch
@@ -299,7 +300,8 @@ def report_folds(dev, stream):
     x, y = streams(2, n, torch.float32, dev)
     x.normal_()
     cases = [("global 2^28", 1, n, 1), ("dim1 16384x8192", 1, 16384, 8192), ("dim2 16384x8192", 16384, 8192, 1),
-             ("dim2 64x2^21", 64, 1 << 21, 1), ("dim2 256x4096x256", 256, 4096, 256)]
+             ("dim2 64x2^21", 64, 1 << 21, 1), ("dim2 4x2^26", 4, 1 << 26, 1),
+             ("dim2 256x4096x256", 256, 4096, 256)]
     for name, lo, mid, hi in cases:
         rec = {"report": "folds", "shape": name, "lo_mid_hi": [lo, mid, hi]}
         out = torch.empty(lo * hi, dtype=torch.float32, device=dev)

@@ -29,7 +29,9 @@
 //  * column kernel (lo > 1, or short runs): one lane per (output, segment)
 //    running the recurrence — or, when lo % W == 0, per group of W adjacent
 //    outputs with 16-byte loads and stores; neighbouring lanes touch
-//    neighbouring addresses;
+//    neighbouring addresses; very narrow tables (at most four such groups)
+//    use the tile kernel, whose waves read 16 or more adjacent rows per load
+//    and scan across them with shuffles;
 //  * long runs split into segments: pass 1 computes each segment's total,
 //    pass 2 the exclusive prefix of the segment totals per run (one
 //    workgroup per run; one lane per output on the column paths), pass 3
@@ -62,6 +64,9 @@ template <class T> __device__ __forceinline__ void two_sum_s(T a, T b, T& s, T& 
 __device__ __forceinline__ float shfl_up_t(float v, int d) { return __shfl_up(v, d, 64); }
 __device__ __forceinline__ double shfl_up_t(double v, int d) { return __shfl_up(v, d, 64); }
 __device__ __forceinline__ int shfl_up_t(int v, int d) { return __shfl_up(v, d, 64); }
+__device__ __forceinline__ float shfl_t(float v, int src) { return __shfl(v, src, 64); }
+__device__ __forceinline__ double shfl_t(double v, int src) { return __shfl(v, src, 64); }
+__device__ __forceinline__ int shfl_t(int v, int src) { return __shfl(v, src, 64); }
 
 // ------------------------------------------------------------- cumsum
 template <class T> struct MSum {
@@ -80,6 +85,7 @@ template <class T> struct MSum {
   }
   static constexpr bool kComm = true;  // the total may be accumulated in any fixed order
   __device__ static V shfl_up(const V& v, int d) { return V{shfl_up_t(v.s, d), shfl_up_t(v.c, d)}; }
+  __device__ static V shfl(const V& v, int src) { return V{shfl_t(v.s, src), shfl_t(v.c, src)}; }
   __device__ static C cident() { return C{0.0, 0.0}; }
   __device__ static C ccomb(const C& a, const C& b) {
     double t, e;
@@ -116,6 +122,7 @@ template <class T> struct MProd {
   __device__ static V comb(const V& a, const V& b) { return norm(V{a.m * b.m, a.e + b.e}); }
   static constexpr bool kComm = true;
   __device__ static V shfl_up(const V& v, int d) { return V{shfl_up_t(v.m, d), shfl_up_t(v.e, d)}; }
+  __device__ static V shfl(const V& v, int src) { return V{shfl_t(v.m, src), shfl_t(v.e, src)}; }
   __device__ static C cident() { return ident(); }
   __device__ static C cadd(const C& c, const V& v) { return comb(c, v); }
   __device__ static C ccomb(const C& a, const C& b) { return comb(a, b); }
@@ -146,6 +153,7 @@ template <bool MAX, class T> struct MExt {
   }
   static constexpr bool kComm = false;  // a NaN resets: only ordered combines
   __device__ static V shfl_up(const V& v, int d) { return V{shfl_up_t(v.nan, d), shfl_up_t(v.acc, d)}; }
+  __device__ static V shfl(const V& v, int src) { return V{shfl_t(v.nan, src), shfl_t(v.acc, src)}; }
   __device__ static C cident() { return ident(); }
   __device__ static C cadd(const C& c, const V& v) { return comb(c, v); }
   __device__ static C ccomb(const C& a, const C& b) { return comb(a, b); }
@@ -371,6 +379,77 @@ __global__ __launch_bounds__(kBlock) void scan_colv_kernel(SGeo<T> g, int64_t se
   }
 }
 
+// Column kernel for very narrow tables (lo % W == 0 and at most four groups
+// of W outputs): a wave is R = 64 / P row phases of P lanes (P a power of
+// two).
+// Lane (r, p) holds the W outputs of group q = qb * P + p at row mb + r of
+// the wave's segment, so one load instruction of the wave reads R adjacent
+// rows (1 KiB of consecutive addresses when the groups cover the rows); the
+// R rows of a step are scanned across the phases with shuffles (lower rows
+// on the left), and the step total — the last phase's inclusive value — is
+// carried to the next step.  Four steps are loaded before the first is
+// scanned.  A wave past the last (segment, group block) leaves at once: no
+// workgroup barriers.  PASS as for the other column kernels.
+template <int OP, class T, int PASS>
+__global__ __launch_bounds__(kBlock) void scan_tile_kernel(SGeo<T> g, int lgP, int64_t qblocks, int64_t segs,
+                                                           int64_t seglen, SSumm* summ) {
+  using M = typename SFam<OP, T>::type;
+  using V = typename M::V;
+  using C = typename M::C;
+  using VT = typename Vec16<T>::type;
+  constexpr int W = Vec16<T>::n;
+  constexpr int U = 4;
+  const int P = 1 << lgP, R = 64 >> lgP;
+  const int lane = threadIdx.x & 63;
+  const int p = lane & (P - 1), r = lane >> lgP;
+  const int last = (R - 1) * P + p;  // this group's last phase
+  const int64_t wv = (int64_t)blockIdx.x * kWaves + (threadIdx.x >> 6);
+  if (wv >= qblocks * segs) return;
+  const int64_t s = wv / qblocks, q = (wv - s * qblocks) * P + p;
+  const int64_t nout = g.lo * g.hi;
+  const bool live = q < nout / W;
+  const int64_t o = W * q;
+  const int64_t h = o / g.lo, i = o - h * g.lo;
+  const int64_t off = i + g.lo * g.mid * h;
+  const VT* px = reinterpret_cast<const VT*>(g.x + off);
+  VT* py = reinterpret_cast<VT*>(g.y + off);
+  const int64_t step = g.lo / W;
+  const int64_t m0 = s * seglen;
+  const int64_t m1 = (m0 + seglen < g.mid) ? m0 + seglen : g.mid;
+  C c[W];
+#pragma unroll
+  for (int k = 0; k < W; ++k) c[k] = (PASS == 3 && live) ? M::unpack(summ[s * nout + o + k]) : M::cident();
+  for (int64_t mb = m0; mb < m1; mb += U * R) {
+    VT v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t m = mb + u * R + r;
+      v[u] = (live && m < m1) ? __builtin_nontemporal_load(px + m * step) : VT{};
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t m = mb + u * R + r;
+      const bool ok = live && m < m1;
+      VT res;
+#pragma unroll
+      for (int k = 0; k < W; ++k) {
+        V a = ok ? M::lift(v[u][k]) : M::ident();
+        for (int d = 1; d < R; d <<= 1) {
+          V lower = M::shfl_up(a, d * P);
+          if (r >= d) a = M::comb(lower, a);
+        }
+        res[k] = M::fin(c[k], a);
+        c[k] = M::cadd(c[k], M::shfl(a, last));
+      }
+      if (PASS != 1 && ok) __builtin_nontemporal_store(res, py + m * step);
+    }
+  }
+  if (PASS == 1 && r == 0 && live) {
+#pragma unroll
+    for (int k = 0; k < W; ++k) summ[s * nout + o + k] = M::pack(c[k]);
+  }
+}
+
 // Replace base[s * st], s = s0..s1-1, by their exclusive prefixes starting
 // from pre, in order (eight loads in flight); returns the running total.
 template <class M>
@@ -493,24 +572,47 @@ int scan_launch(const T* x, T* y, int64_t lo, int64_t mid, int64_t hi, nt2h_stre
     return NT2H_OK;
   }
   const bool colv = (lo % W == 0) && aligned16(x, y);
-  const int64_t units = colv ? nout / W : nout;  // lanes per segment
-  // 16-byte lanes: about 2^17 of them, four loads each in flight (8 MiB),
-  // measured to stream strided rows fastest; scalar lanes: 2^18
-  int64_t segs = ((int64_t)1 << (colv ? 17 : 18)) / units;
-  int64_t maxsegs = (mid + 63) / 64;
-  if (segs > maxsegs) segs = maxsegs;
-  if (segs > 4096) segs = 4096;
-  if (segs < 1) segs = 1;
-  int64_t seglen = (mid + segs - 1) / segs;
-  segs = (mid + seglen - 1) / seglen;
-  const int64_t lanes = units * segs;
-  const int64_t blocks = (lanes + kBlock - 1) / kBlock;
+  // very narrow tables (at most four groups of W outputs, where the column
+  // kernel would run out of lanes): the row-phase tile kernel
+  const bool tile = colv && nout / W <= 4;
+  int lgP = 0;
+  int64_t qblocks = 0, segs, seglen, blocks;
+  if (tile) {
+    // about 16384 waves (the shuffle scan across the phases makes the waves
+    // latency-bound: measured fastest between 8192 and 32768), at least
+    // four steps of R rows per segment
+    const int64_t nq = nout / W;
+    while (((int64_t)1 << lgP) < nq) ++lgP;
+    const int64_t R = 64 >> lgP;
+    qblocks = (nq + ((int64_t)1 << lgP) - 1) >> lgP;  // 1: all groups in one wave
+    segs = 16384 / qblocks;
+    const int64_t maxsegs = (mid + 4 * R - 1) / (4 * R);
+    if (segs > maxsegs) segs = maxsegs;
+    if (segs < 1) segs = 1;
+    seglen = (mid + segs - 1) / segs;
+    segs = (mid + seglen - 1) / seglen;
+    blocks = (qblocks * segs + kWaves - 1) / kWaves;
+  } else {
+    const int64_t units = colv ? nout / W : nout;  // lanes per segment
+    // 16-byte lanes: about 2^17 of them, four loads each in flight (8 MiB),
+    // measured to stream strided rows fastest; scalar lanes: 2^18
+    segs = ((int64_t)1 << (colv ? 17 : 18)) / units;
+    int64_t maxsegs = (mid + 63) / 64;
+    if (segs > maxsegs) segs = maxsegs;
+    if (segs > 4096) segs = 4096;
+    if (segs < 1) segs = 1;
+    seglen = (mid + segs - 1) / segs;
+    segs = (mid + seglen - 1) / seglen;
+    blocks = (units * segs + kBlock - 1) / kBlock;
+  }
   if (blocks > 0x7fffffffLL || (segs > 1 && nout > 0x7fffffffLL)) return NT2H_ERR_TOO_LARGE;
   SSumm* summ = nullptr;
 #define NT2H_SCAN_COL(PASS)                                                                                      \
   do {                                                                                                            \
-    if (colv) hipLaunchKernelGGL((scan_colv_kernel<OP, T, PASS>), dim3((unsigned)blocks), dim3(kBlock), 0, s, g, \
-                                 segs, seglen, summ);                                                             \
+    if (tile) hipLaunchKernelGGL((scan_tile_kernel<OP, T, PASS>), dim3((unsigned)blocks), dim3(kBlock), 0, s, g, \
+                                 lgP, qblocks, segs, seglen, summ);                                               \
+    else if (colv) hipLaunchKernelGGL((scan_colv_kernel<OP, T, PASS>), dim3((unsigned)blocks), dim3(kBlock), 0,  \
+                                      s, g, segs, seglen, summ);                                                  \
     else hipLaunchKernelGGL((scan_col_kernel<OP, T, PASS>), dim3((unsigned)blocks), dim3(kBlock), 0, s, g, segs, \
                             seglen, summ);                                                                        \
     NT2H_HIP(hipGetLastError());                                                                                  \

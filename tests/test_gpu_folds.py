@@ -18,7 +18,9 @@ pytestmark = pytest.mark.gpu
 # (column kernel), outer folds with and without segments, NT2's 3x5x2 table,
 # and lo % 4 == 0 (the 16-byte column kernels) with and without segments
 SHAPES = [(1, 5000, 37), (1, (1 << 20) + 3, 1), (1, 1 << 18, 2), (1, 100, 1000), (1, 700, 3), (7, 3000, 5),
-          (1000, 3, 7), (3, 5, 2), (2, 40000, 1), (1, 1, 9), (16, 5000, 3), (1024, 300, 2)]
+          (1000, 3, 7), (3, 5, 2), (2, 40000, 1), (1, 1, 9), (16, 5000, 3), (1024, 300, 2),
+          # narrow tables: 1, 4, 16 and 15 groups of four outputs (row-phase tile kernels)
+          (4, 30000, 1), (8, 20000, 2), (64, 4097, 1), (12, 777, 5), (12, 5000, 1)]
 
 
 def _dev_tensor(a, dev):
@@ -250,7 +252,7 @@ def test_scan_unit_vectors(dev, ref):
 
 # ------------------------------------------------------------ double tables
 F64_SHAPES = [(1, 5000, 37), (1, (1 << 19) + 3, 1), (1, 1 << 17, 2), (1, 100, 1000), (6, 3000, 5), (1024, 300, 2),
-              (3, 5, 2)]
+              (3, 5, 2), (2, 20000, 1), (10, 3001, 3)]
 
 
 @pytest.mark.parametrize("shape", F64_SHAPES)
