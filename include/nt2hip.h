@@ -132,8 +132,11 @@ int nt2h_bs_fwd_f64(int precision, const double* S, const double* K, const doubl
  * NT2 inner_fold with plus/Zero/sum (core/reduction/.../scalar/inner_fold.hpp:24-53,
  * dsl/functions/reduction/run.hpp:30-104); globalsum (core/functions/globalsum.hpp:45)
  * and dot (reduction/functions/expr/dot.hpp:28) over a whole table.
- * Single pass: per-block partials, last-arriving block combines them in FP64
- * in block order.  *out receives the sum as a double. */
+ * One pass over the data: per-block partials combined in FP64 in a fixed
+ * order — by the last-arriving block (small grids), or by a second small
+ * launch over the stored partials (grids of 4096 blocks or more, where the
+ * per-block tickets would cost bandwidth).  The same bits on every call for
+ * a given size.  *out receives the sum as a double. */
 int nt2h_sum_f32(const float* x, int64_t n, int mode, double* out, nt2h_stream_t s);
 int nt2h_dot_f32(const float* x, const float* y, int64_t n, int mode, double* out, nt2h_stream_t s);
 int nt2h_sum_f64(const double* x, int64_t n, double* out, nt2h_stream_t s);

@@ -87,6 +87,7 @@ template <class T> struct MSum {
   __device__ static V shfl_up(const V& v, int d) { return V{shfl_up_t(v.s, d), shfl_up_t(v.c, d)}; }
   __device__ static V shfl(const V& v, int src) { return V{shfl_t(v.s, src), shfl_t(v.c, src)}; }
   __device__ static C cident() { return C{0.0, 0.0}; }
+  __device__ static C cshfl_up(const C& v, int d) { return C{shfl_up_t(v.s, d), shfl_up_t(v.c, d)}; }
   __device__ static C ccomb(const C& a, const C& b) {
     double t, e;
     two_sum_s(a.s, b.s, t, e);
@@ -124,6 +125,7 @@ template <class T> struct MProd {
   __device__ static V shfl_up(const V& v, int d) { return V{shfl_up_t(v.m, d), shfl_up_t(v.e, d)}; }
   __device__ static V shfl(const V& v, int src) { return V{shfl_t(v.m, src), shfl_t(v.e, src)}; }
   __device__ static C cident() { return ident(); }
+  __device__ static C cshfl_up(const C& v, int d) { return shfl_up(v, d); }
   __device__ static C cadd(const C& c, const V& v) { return comb(c, v); }
   __device__ static C ccomb(const C& a, const C& b) { return comb(a, b); }
   __device__ static T fin(const C& c, const V& v) {
@@ -155,6 +157,7 @@ template <bool MAX, class T> struct MExt {
   __device__ static V shfl_up(const V& v, int d) { return V{shfl_up_t(v.nan, d), shfl_up_t(v.acc, d)}; }
   __device__ static V shfl(const V& v, int src) { return V{shfl_t(v.nan, src), shfl_t(v.acc, src)}; }
   __device__ static C cident() { return ident(); }
+  __device__ static C cshfl_up(const C& v, int d) { return shfl_up(v, d); }
   __device__ static C cadd(const C& c, const V& v) { return comb(c, v); }
   __device__ static C ccomb(const C& a, const C& b) { return comb(a, b); }
   __device__ static T fin(const C& c, const V& v) { return comb(c, v).acc; }
@@ -499,34 +502,40 @@ __global__ __launch_bounds__(kBlock) void scan_col_prefix_kernel(int64_t nout, i
   exclusive_range<M>(M::cident(), summ + o, 0, segs, nout);
 }
 
-// Pass 2: per output (one workgroup), replace the segment totals by their
-// exclusive prefixes, in segment order.  The totals of output o sit at
-// o * ostride + s * sstride (run paths: o * segs + s; column paths with few
-// outputs: s * nout + o).
+// Pass 2: per output (one workgroup of kPrefixBlock lanes), replace the
+// segment totals by their exclusive prefixes, in segment order.  Lane l
+// totals a contiguous range of segments (loads batched), the lane totals are
+// scanned in lane order (shuffles within each wave, then the wave totals in
+// wave order through LDS), and each lane rewrites its range from its
+// exclusive prefix.  The totals of output o sit at o * ostride + s * sstride
+// (run paths: o * segs + s; column paths with few outputs: s * nout + o).
+constexpr int kPrefixBlock = 1024;
 template <int OP, class T>
-__global__ __launch_bounds__(kBlock) void scan_prefix_kernel(int64_t segs, SSumm* summ, int64_t ostride,
-                                                             int64_t sstride) {
+__global__ __launch_bounds__(kPrefixBlock) void scan_prefix_kernel(int64_t segs, SSumm* summ, int64_t ostride,
+                                                                   int64_t sstride) {
   using M = typename SFam<OP, T>::type;
   using C = typename M::C;
-  __shared__ SSumm sh[kBlock];
+  constexpr int kW = kPrefixBlock / 64;
+  __shared__ SSumm sh[kW];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int64_t o = blockIdx.x;
   SSumm* base = summ + o * ostride;
-  const int64_t per = (segs + kBlock - 1) / kBlock;
-  const int64_t s0 = threadIdx.x * per;
+  const int64_t per = (segs + kPrefixBlock - 1) / kPrefixBlock;
+  const int64_t s0 = threadIdx.x * per < segs ? threadIdx.x * per : segs;
   const int64_t s1 = (s0 + per < segs) ? s0 + per : segs;
-  C a = total_range<M>(base, s0, s1, sstride);
-  // inclusive prefix of the lane totals in lane order (Hillis-Steele: each
-  // step combines a lower range with the adjacent higher one), then shifted
-  sh[threadIdx.x] = M::pack(a);
-  __syncthreads();
-  for (int d = 1; d < kBlock; d <<= 1) {
-    C lower = ((int)threadIdx.x >= d) ? M::unpack(sh[threadIdx.x - d]) : M::cident();
-    __syncthreads();
-    if ((int)threadIdx.x >= d) sh[threadIdx.x] = M::pack(M::ccomb(lower, M::unpack(sh[threadIdx.x])));
-    __syncthreads();
+  C inc = total_range<M>(base, s0, s1, sstride);
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    C lower = M::cshfl_up(inc, d);
+    if (lane >= d) inc = M::ccomb(lower, inc);
   }
-  C pre = threadIdx.x == 0 ? M::cident() : M::unpack(sh[threadIdx.x - 1]);
-  exclusive_range<M>(pre, base, s0, s1, sstride);
+  if (lane == 63) sh[w] = M::pack(inc);
+  __syncthreads();
+  C wpre = M::cident();
+  for (int ww = 0; ww < w; ++ww) wpre = M::ccomb(wpre, M::unpack(sh[ww]));
+  C ex = M::cshfl_up(inc, 1);
+  if (lane == 0) ex = M::cident();
+  exclusive_range<M>(M::ccomb(wpre, ex), base, s0, s1, sstride);
 }
 
 template <int OP, class T>
@@ -564,8 +573,8 @@ int scan_launch(const T* x, T* y, int64_t lo, int64_t mid, int64_t hi, nt2h_stre
     }
     NT2H_TRY(get_scratch(st, (size_t)blocks * sizeof(SSumm), (void**)&summ));
     NT2H_SCAN_RUN(1);
-    hipLaunchKernelGGL((scan_prefix_kernel<OP, T>), dim3((unsigned)nout), dim3(kBlock), 0, s, segs, summ, segs,
-                       (int64_t)1);
+    hipLaunchKernelGGL((scan_prefix_kernel<OP, T>), dim3((unsigned)nout), dim3(kPrefixBlock), 0, s, segs, summ,
+                       segs, (int64_t)1);
     NT2H_HIP(hipGetLastError());
     NT2H_SCAN_RUN(3);
 #undef NT2H_SCAN_RUN
@@ -627,8 +636,8 @@ int scan_launch(const T* x, T* y, int64_t lo, int64_t mid, int64_t hi, nt2h_stre
     hipLaunchKernelGGL((scan_col_prefix_kernel<OP, T>), dim3((unsigned)((nout + kBlock - 1) / kBlock)), dim3(kBlock),
                        0, s, nout, segs, summ);
   else
-    hipLaunchKernelGGL((scan_prefix_kernel<OP, T>), dim3((unsigned)nout), dim3(kBlock), 0, s, segs, summ, (int64_t)1,
-                       nout);
+    hipLaunchKernelGGL((scan_prefix_kernel<OP, T>), dim3((unsigned)nout), dim3(kPrefixBlock), 0, s, segs, summ,
+                       (int64_t)1, nout);
   NT2H_HIP(hipGetLastError());
   NT2H_SCAN_COL(3);
 #undef NT2H_SCAN_COL
