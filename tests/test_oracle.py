@@ -89,6 +89,39 @@ def test_normcdf_specials(ref):
     assert v[4] == f32(1.0) and v[5] == f32(0.0)
 
 
+def test_normpdf_unit_values(ref):
+    """modules/core/statistics/unit/scalar/normpdf.cpp:30-35 (specials, float and
+    double) and :44-50 (phi on a -6..4 grid, 1 NT2-ULP; phi(0) exactly
+    Invsqrt_2pi)."""
+    d = json.load(open(os.path.join(GOLD, "nt2_unit_values.json")))["normpdf"]
+    for dt, m in ((np.float32, ref.map_f32), (np.float64, ref.map_f64)):
+        v = m("normpdf", np.array([np.nan, np.inf, -np.inf] + d["specials_x"], np.float64))
+        assert np.isnan(v[0]) and v[1] == 0 and v[2] == 0
+        for got, want, tol in zip(v[3:], d["specials_y"], d["specials_tol"]):
+            assert ref.nt2_ulpdist(dt(got), dt(want)) <= tol, (dt, got, want)
+        g = m("normpdf", np.array(d["x"], np.float64))
+        for x, got, want in zip(d["x"], g, d["y"]):
+            assert ref.nt2_ulpdist(dt(got), dt(want)) <= d["tol"], (dt, x, got, want)
+
+
+def test_fastnormcdf_unit_values_and_published_bound(ref):
+    """modules/core/statistics/unit/scalar/fastnormcdf.cpp:26-31 (float), and the
+    Abramowitz & Stegun 26.2.17 bound the functor implements
+    (generic/fastnormcdf.hpp:39-54): |error| < 7.5e-8, here plus the float
+    rounding of the evaluation, against mpmath's ncdf on [-8, 8]."""
+    d = json.load(open(os.path.join(GOLD, "nt2_unit_values.json")))["fastnormcdf"]
+    v = ref.map_f32("fastnormcdf", np.array([np.nan, np.inf, -np.inf] + d["x"], np.float64))
+    assert np.isnan(v[0]) and v[1] == f32(1.0) and v[2] == f32(0.0)
+    for got, want, tol in zip(v[3:], d["y"], d["tol"]):
+        assert ref.nt2_ulpdist(f32(got), f32(want)) <= tol, (got, want)
+    mpmath.mp.prec = 80
+    x = np.random.default_rng(3).uniform(-8, 8, 3000).astype(np.float32)
+    got = ref.map_f32("fastnormcdf", x).astype(np.float64)
+    truth = np.array([float(mpmath.ncdf(mpmath.mpf(float(t)))) for t in x])
+    err = np.abs(got - truth)
+    assert np.max(err - 7.5e-8 - 4 * np.spacing(np.float32(1.0)) * np.abs(truth)) <= 0, np.max(err)
+
+
 def test_erfc_unit_values(ref):
     """modules/core/euler/unit/scalar/erfc.cpp:42-53 (200 NT2-ULP)."""
     d = json.load(open(os.path.join(GOLD, "nt2_unit_values.json")))["erfc"]
