@@ -252,13 +252,55 @@ def report_saxpy(dev, stream, n=1 << 20):
 
     from nt2_amd import SUM_KAHAN, ops, synth
 
+    from nt2_amd import call
+
     x_np, y_np = synth.saxpy_inputs(n)
     x, y = torch.from_numpy(x_np).to(dev), torch.from_numpy(y_np).to(dev)
     t = _events(stream, lambda: ops.saxpy_(2.5, x, y), 50)
     td = _events(stream, lambda: ops.dot(x, y, SUM_KAHAN), 50)
+    # GPU time per call without the host's submission rate: 100 calls
+    # recorded in one HIP graph, replayed
+    out = torch.zeros(1, dtype=torch.float64, device=dev)
+
+    def saxpy():
+        call("nt2h_saxpy_f32", 2.5, x.data_ptr(), y.data_ptr(), n, torch.cuda.current_stream().cuda_stream)
+
+    def sdot():
+        call("nt2h_dot_f32", x.data_ptr(), y.data_ptr(), n, SUM_KAHAN, out.data_ptr(),
+             torch.cuda.current_stream().cuda_stream)
+
+    tg, tdg = _graph_seconds(saxpy), _graph_seconds(sdot)
     return [{"report": "saxpy_sdot", "n": n, "saxpy_us": t * 1e6, "saxpy_gbps": 12 * n / t / 1e9,
              "sdot_us": td * 1e6, "sdot_gbps": 8 * n / td / 1e9,
-             "note": "2^20 elements: launch-latency bound, not a bandwidth figure"}]
+             "graph_saxpy_us": tg * 1e6, "graph_saxpy_gbps": 12 * n / tg / 1e9,
+             "graph_sdot_us": tdg * 1e6, "graph_sdot_gbps": 8 * n / tdg / 1e9,
+             "note": "2^20 elements: launch-latency bound; *_us are back-to-back calls from Python, "
+                     "graph_* the same calls replayed from one HIP graph of 100"}]
+
+
+def _graph_seconds(fn, calls=100, replays=10):
+    """Seconds per call with `calls` calls recorded in one HIP graph and the
+    graph replayed `replays` times (events around the replays)."""
+    import torch
+
+    cs = torch.cuda.Stream()  # the library keeps workspaces per stream:
+    cs.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(cs):  # allocate them on the capture stream first
+        fn()
+    cs.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=cs):
+        for _ in range(calls):
+            fn()
+    g.replay()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(replays):
+        g.replay()
+    b.record()
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / (calls * replays) / 1e3
 
 
 def report_placement(dev, stream, n=1 << 24):

@@ -5,7 +5,9 @@
  * nt2/sdk/functor/site.hpp:50-53): where NT2 would dispatch transform_ and
  * inner_fold_ to a CUDA site (modules/arch/cuda/include/nt2/sdk/cuda/
  * cuda.hpp:22-41 registers none), a caller binds these entry points.  Every
- * entry point evaluates one fused loop nest as one kernel launch.
+ * entry point evaluates one fused loop nest in one pass over its tables (one
+ * kernel launch; long folds and scans add small launches that combine
+ * segment partials).
  *
  * Conventions
  *  - Plain C types only.  Pointers are caller-owned device pointers unless a
@@ -20,6 +22,11 @@
  *    or to ordinary host memory (the call then synchronises the stream).
  *  - Thread safety: calls on distinct streams may run concurrently; the
  *    library keeps one reduction workspace per (device, stream).
+ *  - Stream capture (HIP graphs): the entry points can be recorded and
+ *    replayed; they record kernels only (no memset or copy nodes).  While the
+ *    stream is capturing, result pointers must be device memory, and the
+ *    same call must have run once on that stream beforehand so that its
+ *    workspace and scratch exist (allocation is not capturable).
  */
 #ifndef NT2HIP_H
 #define NT2HIP_H
@@ -41,7 +48,9 @@ typedef enum nt2h_status {
   NT2H_ERR_NO_DEVICE = -4,    /* no HIP device, or device index out of range */
   NT2H_ERR_HIP = -5,          /* a HIP runtime call failed (see nt2h_last_hip_error) */
   NT2H_ERR_OUT_OF_MEMORY = -6,/* device allocation failed */
-  NT2H_ERR_TOO_LARGE = -7     /* size beyond what the entry point supports */
+  NT2H_ERR_TOO_LARGE = -7,    /* size beyond what the entry point supports */
+  NT2H_ERR_CAPTURE = -8       /* the call needs a workspace allocation while its stream is capturing:
+                                 run it once on that stream before the capture */
 } nt2h_status;
 
 /* Elementwise functor ids (the NT2 functors on the path). */

@@ -32,6 +32,7 @@ SCAN_OPS = {"cumsum": 0, "cumprod": 1, "cummax": 2, "cummin": 3}
 STATUS = {
     0: "NT2H_OK", -1: "NT2H_ERR_INVALID_ARG", -2: "NT2H_ERR_NULL_POINTER", -3: "NT2H_ERR_BAD_OP",
     -4: "NT2H_ERR_NO_DEVICE", -5: "NT2H_ERR_HIP", -6: "NT2H_ERR_OUT_OF_MEMORY", -7: "NT2H_ERR_TOO_LARGE",
+    -8: "NT2H_ERR_CAPTURE",
 }
 
 # every exported entry point with its ctypes signature (restype, argtypes)

@@ -244,7 +244,7 @@ int nt2h_bs_fwd_adj_f32(const float* S, const float* K, const float* T, const fl
   bool host = false;
   double* dst = nullptr;
   if (totals) {
-    NT2H_TRY(is_host_pointer(totals, &host));
+    NT2H_TRY(is_host_pointer(totals, &host, s));
     NT2H_TRY(get_workspace(s, &ws));
     dst = host ? ws.result : totals;
   }
@@ -281,7 +281,7 @@ int nt2h_bs_fwd_adj_f64(const double* S, const double* K, const double* T, const
   bool host = false;
   double* dst = nullptr;
   if (totals) {
-    NT2H_TRY(is_host_pointer(totals, &host));
+    NT2H_TRY(is_host_pointer(totals, &host, s));
     NT2H_TRY(get_workspace(s, &ws));
     dst = host ? ws.result : totals;
   }

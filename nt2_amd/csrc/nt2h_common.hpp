@@ -56,6 +56,6 @@ int get_scratch(nt2h_stream_t s, size_t bytes, void** out);
 // Deliver nv doubles computed into dev (device memory) to out (device or
 // host memory); for host memory the stream is synchronised.
 int deliver(nt2h_stream_t s, double* out, const double* dev, int nv, bool out_is_host);
-int is_host_pointer(const void* p, bool* host);
+int is_host_pointer(const void* p, bool* host, nt2h_stream_t s);
 
 }  // namespace nt2h

@@ -203,7 +203,7 @@ static int fold_f32(const float* x, const float* y, int64_t n, int mode, double*
   if (y) NT2H_TRY(check_ptrs(n, x, y));
   else NT2H_TRY(check_ptrs(n, x));
   bool host = false;
-  NT2H_TRY(is_host_pointer(out, &host));
+  NT2H_TRY(is_host_pointer(out, &host, s));
   Workspace ws;
   NT2H_TRY(get_workspace(s, &ws));
   double* dst = host ? ws.result : out;
@@ -234,7 +234,7 @@ static int fold_f64(const double* x, const double* y, int64_t n, double* out, nt
   if (y) NT2H_TRY(check_ptrs(n, x, y));
   else NT2H_TRY(check_ptrs(n, x));
   bool host = false;
-  NT2H_TRY(is_host_pointer(out, &host));
+  NT2H_TRY(is_host_pointer(out, &host, s));
   Workspace ws;
   NT2H_TRY(get_workspace(s, &ws));
   double* dst = host ? ws.result : out;

@@ -38,6 +38,13 @@ std::map<void*, void*> g_colored;  // returned pointer -> hipMalloc base
 unsigned g_next_color = 0;
 }  // namespace
 
+// Allocation cannot be recorded into a graph: a stream that is capturing
+// must already own its workspace and scratch.
+static bool capturing(nt2h_stream_t s) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  return hipStreamIsCapturing(as_hip(s), &cs) == hipSuccess && cs != hipStreamCaptureStatusNone;
+}
+
 int get_workspace(nt2h_stream_t s, Workspace* ws) {
   int dev = 0;
   NT2H_HIP(hipGetDevice(&dev));
@@ -48,6 +55,7 @@ int get_workspace(nt2h_stream_t s, Workspace* ws) {
     *ws = it->second;
     return NT2H_OK;
   }
+  if (capturing(s)) return NT2H_ERR_CAPTURE;
   Workspace w{};
   NT2H_HIP(hipMalloc((void**)&w.partials, (size_t)kMaxPartials * kMaxFoldBlocks * sizeof(double)));
   NT2H_HIP(hipMalloc((void**)&w.gpartials, (size_t)kMaxPartials * kMaxGroups * sizeof(double)));
@@ -80,6 +88,7 @@ int get_scratch(nt2h_stream_t s, size_t bytes, void** out) {
   std::lock_guard<std::mutex> lock(g_scratch_mutex);
   Scratch& sc = g_scratch[std::make_pair(dev, as_hip(s))];
   if (sc.bytes < bytes) {
+    if (capturing(s)) return NT2H_ERR_CAPTURE;
     if (sc.p) {
       NT2H_HIP(hipStreamSynchronize(as_hip(s)));
       NT2H_HIP(hipFree(sc.p));
@@ -94,7 +103,15 @@ int get_scratch(nt2h_stream_t s, size_t bytes, void** out) {
   return NT2H_OK;
 }
 
-int is_host_pointer(const void* p, bool* host) {
+int is_host_pointer(const void* p, bool* host, nt2h_stream_t s) {
+  // Under stream capture the attribute query is refused (and would
+  // invalidate the capture), and a host result would need a stream
+  // synchronisation, which a graph cannot contain: result pointers passed
+  // while the stream is capturing are device memory.
+  if (capturing(s)) {
+    *host = false;
+    return NT2H_OK;
+  }
   hipPointerAttribute_t attr;
   hipError_t e = hipPointerGetAttributes(&attr, p);
   if (e != hipSuccess) {
@@ -152,6 +169,7 @@ const char* nt2h_strerror(int code) {
     case NT2H_ERR_HIP: return "HIP runtime error";
     case NT2H_ERR_OUT_OF_MEMORY: return "out of device memory";
     case NT2H_ERR_TOO_LARGE: return "size too large";
+    case NT2H_ERR_CAPTURE: return "workspace allocation needed while the stream is capturing";
   }
   return "unknown status";
 }
