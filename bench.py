@@ -443,7 +443,7 @@ def main():
                     help="functors of the precision sweep report")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
+        print(f"bench.py: bench.py measures one GPU only (asked for --gpus {args.gpus}); "
               "the evaluator shards by option range with no exchange except the two FP64 totals",
               file=sys.stderr)
         sys.exit(2)

@@ -1,6 +1,6 @@
 // nt2ref — CPU restatement of NT2's *scalar* evaluation path (the oracle).
 //
-// TEST INFRASTRUCTURE ONLY.  This file is the checker the HIP evaluator is
+// TEST INFRASTRUCTURE ONLY.  This file is the reference the HIP evaluator is
 // compared against: only tests/, __graft_entry__.smoke() and bench.py's
 // cpu_baseline leg may build, load or call it.  The product library
 // (nt2_amd/, include/nt2hip.h) never links or falls back to it.
