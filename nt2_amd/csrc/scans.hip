@@ -83,6 +83,9 @@ template <class T> struct MSum {
     two_sum_s(a.s, b.s, t, e);
     return V{t, (a.c + b.c) + e};
   }
+  // combine inside a tile (at most 1024 elements) and the form kept in LDS
+  __device__ static V combs(const V& a, const V& b) { return comb(a, b); }
+  __device__ static V tnorm(const V& v) { return v; }
   static constexpr bool kComm = true;  // the total may be accumulated in any fixed order
   __device__ static V shfl_up(const V& v, int d) { return V{shfl_up_t(v.s, d), shfl_up_t(v.c, d)}; }
   __device__ static V shfl(const V& v, int src) { return V{shfl_t(v.s, src), shfl_t(v.c, src)}; }
@@ -121,6 +124,12 @@ template <class T> struct MProd {
   __device__ static V ident() { return V{1.0, 0}; }
   __device__ static V lift(T x) { return norm(V{(double)x, 0}); }
   __device__ static V comb(const V& a, const V& b) { return norm(V{a.m * b.m, a.e + b.e}); }
+  // Inside a tile a chain multiplies at most 256 normalised mantissas per
+  // wave (>= 2^-256) and three normalised wave totals, so the product stays
+  // a normal double without renormalising; wave totals are renormalised
+  // before they are combined across waves.
+  __device__ static V combs(const V& a, const V& b) { return V{a.m * b.m, a.e + b.e}; }
+  __device__ static V tnorm(const V& v) { return norm(v); }
   static constexpr bool kComm = true;
   __device__ static V shfl_up(const V& v, int d) { return V{shfl_up_t(v.m, d), shfl_up_t(v.e, d)}; }
   __device__ static V shfl(const V& v, int src) { return V{shfl_t(v.m, src), shfl_t(v.e, src)}; }
@@ -153,6 +162,8 @@ template <bool MAX, class T> struct MExt {
     const bool keep_a = MAX ? (a.acc > b.acc) : (a.acc < b.acc);
     return V{a.nan, keep_a ? a.acc : b.acc};
   }
+  __device__ static V combs(const V& a, const V& b) { return comb(a, b); }
+  __device__ static V tnorm(const V& v) { return v; }
   static constexpr bool kComm = false;  // a NaN resets: only ordered combines
   __device__ static V shfl_up(const V& v, int d) { return V{shfl_up_t(v.nan, d), shfl_up_t(v.acc, d)}; }
   __device__ static V shfl(const V& v, int src) { return V{shfl_t(v.nan, src), shfl_t(v.acc, src)}; }
@@ -179,7 +190,7 @@ template <class M> __device__ __forceinline__ typename M::V wave_inscan(typename
 #pragma unroll
   for (int d = 1; d < 64; d <<= 1) {
     typename M::V o = M::shfl_up(v, d);
-    if (lane >= d) v = M::comb(o, v);
+    if (lane >= d) v = M::combs(o, v);
   }
   return v;
 }
@@ -230,29 +241,29 @@ __device__ __forceinline__ typename M::C run_segment(const SGeo<T>& g, int64_t o
 #pragma unroll
     for (int k = 0; k < W; ++k) {
       V xk = (m + k < m1) ? M::lift(cur[k]) : M::ident();
-      acc = (k == 0) ? xk : M::comb(acc, xk);
+      acc = (k == 0) ? xk : M::combs(acc, xk);
       p[k] = acc;
     }
     if (kLaneTotals) {
       lanes = M::comb(lanes, acc);
     } else {
       V inc = wave_inscan<M>(acc);
-      if (lane == 63) lds[par][wave] = inc;
+      if (lane == 63) lds[par][wave] = M::tnorm(inc);
       __syncthreads();
       V wpre = M::ident(), tot = M::ident();
 #pragma unroll
       for (int w = 0; w < kWaves; ++w) {
         V t = lds[par][w];
-        if (w < wave) wpre = M::comb(wpre, t);
-        tot = M::comb(tot, t);
+        if (w < wave) wpre = M::combs(wpre, t);
+        tot = M::combs(tot, t);
       }
       if (WRITE) {
         V ex = M::shfl_up(inc, 1);
         if (lane == 0) ex = M::ident();
-        ex = M::comb(wpre, ex);
+        ex = M::combs(wpre, ex);
         T r[W];
 #pragma unroll
-        for (int k = 0; k < W; ++k) r[k] = M::fin(carry, M::comb(ex, p[k]));
+        for (int k = 0; k < W; ++k) r[k] = M::fin(carry, M::combs(ex, p[k]));
         if (VEC && m + W - 1 < m1) {
           VT q;
 #pragma unroll
@@ -272,7 +283,7 @@ __device__ __forceinline__ typename M::C run_segment(const SGeo<T>& g, int64_t o
   }
   if (kLaneTotals) {
     V inc = wave_inscan<M>(lanes);
-    if (lane == 63) lds[par][wave] = inc;
+    if (lane == 63) lds[par][wave] = M::tnorm(inc);
     __syncthreads();
     V tot = M::ident();
 #pragma unroll
@@ -439,7 +450,7 @@ __global__ __launch_bounds__(kBlock) void scan_tile_kernel(SGeo<T> g, int lgP, i
         V a = ok ? M::lift(v[u][k]) : M::ident();
         for (int d = 1; d < R; d <<= 1) {
           V lower = M::shfl_up(a, d * P);
-          if (r >= d) a = M::comb(lower, a);
+          if (r >= d) a = M::combs(lower, a);
         }
         res[k] = M::fin(c[k], a);
         c[k] = M::cadd(c[k], M::shfl(a, last));

@@ -240,6 +240,20 @@ def test_cumprod_accuracy(dev, shape):
     assert np.all(np.abs(got - want) <= 2 * np.spacing(want.astype(np.float32)))
 
 
+@pytest.mark.parametrize("shape", [(1, (1 << 20) + 3, 1), (1, 5000, 37), (16, 5000, 3), (4, 30000, 1)])
+def test_cumprod_powers_of_two_exact(dev, ref, shape):
+    """Long products of 0.5 and of 2: every prefix is an exact power of two
+    down to the smallest subnormal and then 0 (or up to FLT_MAX's exponent
+    and then inf) — NT2's sequential float recurrence gives the same bits,
+    however the products are grouped inside tiles and across segments."""
+    lo, mid, hi = shape
+    for v in (0.5, 2.0):
+        x = np.full(lo * mid * hi, v, np.float32)
+        got = _scan("cumprod", x, lo, mid, hi, dev)
+        want = ref.scan_lmh("cumprod", x, lo, mid, hi)
+        assert _bits_eq(got, want).all(), v
+
+
 def test_scan_unit_vectors(dev, ref):
     """modules/core/swar/unit/table/cumsum.cpp: reshape(_(1, 30), 3, 5, 2) along 1 and 2."""
     a = np.arange(1, 31, dtype=np.float32)
