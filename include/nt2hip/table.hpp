@@ -78,24 +78,69 @@ struct extent {
 template <class T> class table;
 
 // ============================================================ nodes
+// Four consecutive elements of a loop nest (the vector body of the kernels).
+template <class T> struct quad {
+  T v[4];
+};
+
 namespace node {
 // Every node evaluates at(i, rows): linear index i of a column-major
-// (rows x cols) loop nest (rows lets an sx-broadcast row find its column).
+// (rows x cols) loop nest (rows lets an sx-broadcast row find its column),
+// and at4(i, rows): elements i..i+3 for i a multiple of 4, with 16-byte loads
+// of the tables.  aligned() says whether every table of the tree starts on
+// a 16-byte boundary (the condition for at4).
 template <class T> struct term {  // a table read at the element index
   const T* p;
   template <class P> NT2H_FD T at(int64_t i, int64_t) const { return p[i]; }
+  template <class P> NT2H_FD quad<T> at4(int64_t i, int64_t) const {
+    using V = typename nt2h::Vec16<T>::type;
+    constexpr int W = nt2h::Vec16<T>::n;
+    quad<T> q;
+#pragma unroll
+    for (int h = 0; h < 4 / W; ++h) {
+      V v = __builtin_nontemporal_load(reinterpret_cast<const V*>(p + i) + h);
+#pragma unroll
+      for (int k = 0; k < W; ++k) q.v[h * W + k] = v[k];
+    }
+    return q;
+  }
+  bool aligned() const { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
 };
 template <class T> struct row_bcast {  // sx: a 1 x cols row repeated down every column
   const T* p;
   template <class P> NT2H_FD T at(int64_t i, int64_t rows) const { return p[i / rows]; }
+  template <class P> NT2H_FD quad<T> at4(int64_t i, int64_t rows) const {
+    quad<T> q;
+    const int64_t c0 = i / rows;
+    if ((i + 3) / rows == c0) {  // the four elements share a column
+      const T v = p[c0];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) q.v[k] = v;
+    } else {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) q.v[k] = p[(i + k) / rows];
+    }
+    return q;
+  }
+  bool aligned() const { return true; }
 };
 template <class T> struct scalar {
   T v;
   template <class P> NT2H_FD T at(int64_t, int64_t) const { return v; }
+  template <class P> NT2H_FD quad<T> at4(int64_t, int64_t) const { return quad<T>{{v, v, v, v}}; }
+  bool aligned() const { return true; }
 };
 template <class F, class A> struct unary {
   A a;
   template <class P> NT2H_FD auto at(int64_t i, int64_t r) const { return F::template apply<P>(a.template at<P>(i, r)); }
+  template <class P> NT2H_FD auto at4(int64_t i, int64_t r) const {
+    const auto x = a.template at4<P>(i, r);
+    quad<decltype(F::template apply<P>(x.v[0]))> q;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) q.v[k] = F::template apply<P>(x.v[k]);
+    return q;
+  }
+  bool aligned() const { return a.aligned(); }
 };
 template <class F, class A, class B> struct binary {
   A a;
@@ -103,6 +148,15 @@ template <class F, class A, class B> struct binary {
   template <class P> NT2H_FD auto at(int64_t i, int64_t r) const {
     return F::template apply<P>(a.template at<P>(i, r), b.template at<P>(i, r));
   }
+  template <class P> NT2H_FD auto at4(int64_t i, int64_t r) const {
+    const auto x = a.template at4<P>(i, r);
+    const auto y = b.template at4<P>(i, r);
+    quad<decltype(F::template apply<P>(x.v[0], y.v[0]))> q;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) q.v[k] = F::template apply<P>(x.v[k], y.v[k]);
+    return q;
+  }
+  bool aligned() const { return a.aligned() && b.aligned(); }
 };
 }  // namespace node
 
@@ -272,6 +326,29 @@ template <class P, class T, class N>
 __global__ __launch_bounds__(256) void k_transform(N n, T* out, int64_t numel, int64_t rows) {
   const int64_t stride = (int64_t)gridDim.x * 256;
   for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < numel; i += stride) out[i] = n.template at<P>(i, rows);
+}
+// Vector body: lane l evaluates elements 4q..4q+3 of quad q (16-byte loads
+// and stores, one quad per lane and step); the last numel % 4 elements go
+// through at().
+template <class P, class T, class N>
+__global__ __launch_bounds__(256) void k_transform4(N n, T* out, int64_t numel, int64_t rows) {
+  using V = typename nt2h::Vec16<T>::type;
+  constexpr int W = nt2h::Vec16<T>::n;
+  const int64_t nq = numel / 4;
+  const int64_t stride = (int64_t)gridDim.x * 256;
+  const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  for (int64_t q = g; q < nq; q += stride) {
+    const quad<T> r = n.template at4<P>(4 * q, rows);
+#pragma unroll
+    for (int h = 0; h < 4 / W; ++h) {
+      V v;
+#pragma unroll
+      for (int k = 0; k < W; ++k) v[k] = r.v[h * W + k];
+      __builtin_nontemporal_store(v, reinterpret_cast<V*>(out + 4 * q) + h);
+    }
+  }
+  const int64_t t = 4 * nq + g;
+  if (t < numel) out[t] = n.template at<P>(t, rows);
 }
 template <class P, class T, class N1, class N2>
 __global__ __launch_bounds__(256) void k_tie(N1 n1, N2 n2, T* o1, T* o2, int64_t numel, int64_t rows) {
@@ -445,11 +522,21 @@ template <class T> class table {
 namespace detail {
 template <class P> constexpr int precision_id() { return P::fused ? NT2H_FUSED : NT2H_EXACT; }
 
-// generic: one kernel for the tree
+// generic: one kernel for the tree — the vector body when the output and
+// every table of the tree are 16-byte aligned (one quad per lane: the grid
+// covers the table), else the scalar body
 template <class P, class T, class N> inline void run_generic(T* out, const N& n, extent x) {
   if (x.numel() == 0) return;
-  hipLaunchKernelGGL((k_transform<P, T, N>), dim3(grid_for(x.numel())), dim3(256), 0, current_stream(), n, out,
-                     x.numel(), x.rows);
+  if (n.aligned() && (reinterpret_cast<uintptr_t>(out) & 15u) == 0) {
+    int64_t b = (x.numel() / 4 + 255) / 256;
+    if (b < 1) b = 1;
+    if (b > (int64_t(1) << 30)) b = int64_t(1) << 30;
+    hipLaunchKernelGGL((k_transform4<P, T, N>), dim3((unsigned)b), dim3(256), 0, current_stream(), n, out,
+                       x.numel(), x.rows);
+  } else {
+    hipLaunchKernelGGL((k_transform<P, T, N>), dim3(grid_for(x.numel())), dim3(256), 0, current_stream(), n, out,
+                       x.numel(), x.rows);
+  }
   hip_check(hipGetLastError(), "transform launch");
 }
 template <class P, class T, class N> struct lower {

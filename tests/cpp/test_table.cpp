@@ -12,6 +12,7 @@
 
 extern "C" {
 void nt2ref_map_f32(int op, const float* x, float* y, int64_t n);
+void nt2ref_map_f64(int op, const double* x, double* y, int64_t n);
 void nt2ref_saxpy_f32(float a, const float* x, float* y, int64_t n);
 void nt2ref_bs_fwd_f32(const float*, const float*, const float*, const float*, const float*, float*, float*, int64_t);
 void nt2ref_exact_dot_f32(const float* x, const float* y, int64_t n, double* out);
@@ -133,6 +134,37 @@ int main() {
     }
     report("colfold_sum_dim1_sx", ok1);
     report("rowfold_sum_dim2", ok2);
+  }
+  {  // elementwise trees through the vector body: sx rows that straddle a quad
+     // (37 rows), a scalar, a quad-free tail, and a double table — bit-exact vs
+     // the same two-rounding composition on the CPU
+    const int64_t rows = 37, cols = 7;
+    auto hm = uniform(rows * cols, -3.f, 3.f, 13), hs = uniform(cols, 0.5f, 2.f, 14);
+    table<float> M(hm, rows, cols), Sx(hs, 1, cols);
+    table<float> Z;
+    Z = M * nt2hip::sx(Sx) + exp(M) - 0.5f;
+    std::vector<float> e(rows * cols), want(rows * cols);
+    nt2ref_map_f32(0, hm.data(), e.data(), rows * cols);
+    for (int64_t i = 0; i < rows * cols; ++i) {
+      float a = hm[i] * hs[i / rows];
+      float b = a + e[i];
+      want[i] = b - 0.5f;
+    }
+    report("vector_tree_sx_straddling_quads", same_bits(Z.to_host(), want));
+    const int64_t nd = 4099;
+    std::vector<double> hd(nd);
+    for (int64_t i = 0; i < nd; ++i) hd[i] = -4.0 + 8.0 * (double)i / (double)nd;
+    table<double> D(hd, nd, 1);
+    table<double> E;
+    E = exp(D) * D + 1.0;
+    std::vector<double> ed(nd), wd(nd);
+    nt2ref_map_f64(0, hd.data(), ed.data(), nd);
+    for (int64_t i = 0; i < nd; ++i) {
+      double a = ed[i] * hd[i];
+      wd[i] = a + 1.0;
+    }
+    auto gd = E.to_host();
+    report("vector_tree_f64", std::memcmp(gd.data(), wd.data(), sizeof(double) * nd) == 0);
   }
   {  // Black-Scholes through tie (5 -> 2 streams): bit-exact vs NT2's scalar composition
     const int64_t m = 100003;
