@@ -350,6 +350,36 @@ __global__ __launch_bounds__(256) void k_transform4(N n, T* out, int64_t numel, 
   const int64_t t = 4 * nq + g;
   if (t < numel) out[t] = n.template at<P>(t, rows);
 }
+// tie's vector body: both children evaluated on a quad before either output
+// quad is stored
+template <class P, class T, class N1, class N2>
+__global__ __launch_bounds__(256) void k_tie4(N1 n1, N2 n2, T* o1, T* o2, int64_t numel, int64_t rows) {
+  using V = typename nt2h::Vec16<T>::type;
+  constexpr int W = nt2h::Vec16<T>::n;
+  const int64_t nq = numel / 4;
+  const int64_t stride = (int64_t)gridDim.x * 256;
+  const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  for (int64_t q = g; q < nq; q += stride) {
+    const quad<T> a = n1.template at4<P>(4 * q, rows), b = n2.template at4<P>(4 * q, rows);
+#pragma unroll
+    for (int h = 0; h < 4 / W; ++h) {
+      V va, vb;
+#pragma unroll
+      for (int k = 0; k < W; ++k) {
+        va[k] = a.v[h * W + k];
+        vb[k] = b.v[h * W + k];
+      }
+      __builtin_nontemporal_store(va, reinterpret_cast<V*>(o1 + 4 * q) + h);
+      __builtin_nontemporal_store(vb, reinterpret_cast<V*>(o2 + 4 * q) + h);
+    }
+  }
+  const int64_t t = 4 * nq + g;
+  if (t < numel) {
+    T a = n1.template at<P>(t, rows), b = n2.template at<P>(t, rows);
+    o1[t] = a;
+    o2[t] = b;
+  }
+}
 template <class P, class T, class N1, class N2>
 __global__ __launch_bounds__(256) void k_tie(N1 n1, N2 n2, T* o1, T* o2, int64_t numel, int64_t rows) {
   const int64_t stride = (int64_t)gridDim.x * 256;
@@ -773,8 +803,16 @@ template <class T> struct tie2 {
     // (core/functions/expr/tie.hpp:80-115): outputs may alias inputs
     table<T> ta(x.rows, x.cols), tb(x.rows, x.cols);
     if (x.numel() > 0) {
-      hipLaunchKernelGGL((detail::k_tie<exact, T, N1, N2>), dim3(detail::grid_for(x.numel())), dim3(256), 0,
-                         current_stream(), es.e1.n, es.e2.n, ta.data(), tb.data(), x.numel(), x.rows);
+      if (es.e1.n.aligned() && es.e2.n.aligned()) {  // the outputs are fresh tables: aligned
+        int64_t nb = (x.numel() / 4 + 255) / 256;
+        if (nb < 1) nb = 1;
+        if (nb > (int64_t(1) << 30)) nb = int64_t(1) << 30;
+        hipLaunchKernelGGL((detail::k_tie4<exact, T, N1, N2>), dim3((unsigned)nb), dim3(256), 0, current_stream(),
+                           es.e1.n, es.e2.n, ta.data(), tb.data(), x.numel(), x.rows);
+      } else {
+        hipLaunchKernelGGL((detail::k_tie<exact, T, N1, N2>), dim3(detail::grid_for(x.numel())), dim3(256), 0,
+                           current_stream(), es.e1.n, es.e2.n, ta.data(), tb.data(), x.numel(), x.rows);
+      }
       hip_check(hipGetLastError(), "tie launch");
     }
     a = std::move(ta);
