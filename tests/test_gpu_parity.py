@@ -254,6 +254,39 @@ def test_sum_ill_conditioned_kahan(dev, ref):
     assert ops.sum(ga, nt2_amd.SUM_KAHAN).item() == k
 
 
+@pytest.mark.parametrize("n,unaligned,host_out", [((1 << 22) + 3, False, True), ((1 << 21) + 1, True, False),
+                                                  ((1 << 21) + 1, True, True)])
+def test_sum_dot_two_launch_paths(dev, ref, n, unaligned, host_out):
+    """Grids of 4096 workgroups or more finish in a second launch: 16-byte and
+    scalar bodies, results to device memory and to host memory."""
+    import ctypes as C
+
+    import torch
+
+    a, b = synth.sumdot_inputs(n)
+    if unaligned:
+        ga = _gpu(np.concatenate([[0], a]).astype(np.float32), dev)[1:]
+        gb = _gpu(np.concatenate([[0], b]).astype(np.float32), dev)[1:]
+        assert ga.data_ptr() % 16 != 0
+    else:
+        ga, gb = _gpu(a, dev), _gpu(b, dev)
+    st = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    res = []
+    for fn, args in (("nt2h_sum_f32", (ga.data_ptr(), n)), ("nt2h_dot_f32", (ga.data_ptr(), gb.data_ptr(), n))):
+        if host_out:
+            out = (C.c_double * 1)()
+            nt2_amd.call(fn, *args, nt2_amd.SUM_KAHAN, C.cast(out, C.c_void_p), st)
+            res.append(out[0])
+        else:
+            d = torch.zeros(1, dtype=torch.float64, device=dev)
+            nt2_amd.call(fn, *args, nt2_amd.SUM_KAHAN, d.data_ptr(), st)
+            res.append(d.item())
+    hi, lo = ref.exact_sum_f32(a)
+    assert abs(res[0] - (hi + lo)) <= 1e-12 * abs(hi)
+    dh, dl = ref.exact_dot_f32(a, b)
+    assert abs(res[1] - (dh + dl)) <= 1e-12 * abs(dh)
+
+
 def test_sum_host_output_pointer(dev, ref):
     import ctypes as C
 
