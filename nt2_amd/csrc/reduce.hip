@@ -148,13 +148,50 @@ __global__ __launch_bounds__(kBlock) void fold_f32_kernel(const float* __restric
   if (grid_combine<1>(bsum, ws, res)) out[0] = res[0];
 }
 
-template <bool DOT>
+// Double tables: plain FP64 lane sums (16-byte loads when aligned, two
+// vectors in flight), the same block tree and combine as the float kernel.
+template <bool DOT, bool VEC, bool TWO>
 __global__ __launch_bounds__(kBlock) void fold_f64_kernel(const double* __restrict__ x,
                                                           const double* __restrict__ y, int64_t n,
                                                           Workspace ws, double* out) {
   double s = 0.0;
   const int64_t stride = (int64_t)gridDim.x * kBlock;
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+  const int64_t gid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  int64_t done = 0;
+  if (VEC) {
+    const int64_t nv = n / 2;
+    const f64x2* x2 = reinterpret_cast<const f64x2*>(x);
+    const f64x2* y2 = reinterpret_cast<const f64x2*>(y);
+    int64_t i = gid;
+    for (; i + stride < nv; i += 2 * stride) {  // two vectors in flight
+      f64x2 a0 = __builtin_nontemporal_load(x2 + i), a1 = __builtin_nontemporal_load(x2 + i + stride);
+      if (DOT) {
+        f64x2 b0 = __builtin_nontemporal_load(y2 + i), b1 = __builtin_nontemporal_load(y2 + i + stride);
+        s += a0.x * b0.x;
+        s += a0.y * b0.y;
+        s += a1.x * b1.x;
+        s += a1.y * b1.y;
+      } else {
+        s += a0.x;
+        s += a0.y;
+        s += a1.x;
+        s += a1.y;
+      }
+    }
+    for (; i < nv; i += stride) {
+      f64x2 a = __builtin_nontemporal_load(x2 + i);
+      if (DOT) {
+        f64x2 b = __builtin_nontemporal_load(y2 + i);
+        s += a.x * b.x;
+        s += a.y * b.y;
+      } else {
+        s += a.x;
+        s += a.y;
+      }
+    }
+    done = nv * 2;
+  }
+  for (int64_t i = done + gid; i < n; i += stride) {
     double a = __builtin_nontemporal_load(x + i);
     if (DOT) s += a * __builtin_nontemporal_load(y + i);
     else s += a;
@@ -169,6 +206,10 @@ __global__ __launch_bounds__(kBlock) void fold_f64_kernel(const double* __restri
     double t = lds[0];
     for (int w = 1; w < kWaves; ++w) t += lds[w];
     bsum[0] = t;
+  }
+  if (TWO) {
+    if (threadIdx.x == 0) ws.bpartials[blockIdx.x] = bsum[0];
+    return;
   }
   double res[1] = {0.0};
   if (grid_combine<1>(bsum, ws, res)) out[0] = res[0];
@@ -238,11 +279,23 @@ static int fold_f64(const double* x, const double* y, int64_t n, double* out, nt
   Workspace ws;
   NT2H_TRY(get_workspace(s, &ws));
   double* dst = host ? ws.result : out;
-  unsigned g = fold_blocks(n);
-  if (y)
-    hipLaunchKernelGGL((fold_f64_kernel<true>), dim3(g), dim3(kBlock), 0, as_hip(s), x, y, n, ws, dst);
-  else
-    hipLaunchKernelGGL((fold_f64_kernel<false>), dim3(g), dim3(kBlock), 0, as_hip(s), x, y, n, ws, dst);
+  // 16-byte loads for sums; the dot reads 8-byte elements of its two
+  // streams (measured 10% faster for it than 16-byte pairs)
+  const bool vec = !y && aligned16(x);
+  const unsigned g = fold_blocks(vec ? (n + 1) / 2 : n);
+  const bool two = g >= kTwoLaunchBlocks;
+  hipStream_t st = as_hip(s);
+#define NT2H_F64_FOLD(DOT, VEC, TWO) \
+  hipLaunchKernelGGL((fold_f64_kernel<DOT, VEC, TWO>), dim3(g), dim3(kBlock), 0, st, x, y, n, ws, dst)
+  if (y) {
+    if (two) NT2H_F64_FOLD(true, false, true);
+    else NT2H_F64_FOLD(true, false, false);
+  } else {
+    if (vec) { if (two) NT2H_F64_FOLD(false, true, true); else NT2H_F64_FOLD(false, true, false); }
+    else { if (two) NT2H_F64_FOLD(false, false, true); else NT2H_F64_FOLD(false, false, false); }
+  }
+#undef NT2H_F64_FOLD
+  if (two) hipLaunchKernelGGL((fold_partials_kernel<1>), dim3(partials_blocks(g)), dim3(kBlock), 0, st, (int)g, ws, dst);
   NT2H_HIP(hipGetLastError());
   return deliver(s, out, ws.result, 1, host);
 }

@@ -301,11 +301,20 @@ def test_sum_host_output_pointer(dev, ref):
     assert abs(out[0] - (hi + lo)) <= 1e-12 * abs(hi)
 
 
-def test_sum_dot_f64(dev, ref):
-    a, b = synth.sumdot_inputs(1 << 20)
+@pytest.mark.parametrize("n,unaligned", [(1 << 20, False), ((1 << 22) + 1, False), ((1 << 22) + 1, True)])
+def test_sum_dot_f64(dev, ref, n, unaligned):
+    """Double tables: 16-byte and scalar bodies, one launch (2^20) and two
+    (grids of 4096 workgroups or more)."""
+    a, b = synth.sumdot_inputs(n)
     a64, b64 = a.astype(np.float64), b.astype(np.float64)
-    s = ops.sum(_gpu(a64, dev)).item()
-    d = ops.dot(_gpu(a64, dev), _gpu(b64, dev)).item()
+    if unaligned:
+        ga = _gpu(np.concatenate([[0.0], a64]), dev)[1:]
+        gb = _gpu(np.concatenate([[0.0], b64]), dev)[1:]
+        assert ga.data_ptr() % 16 != 0
+    else:
+        ga, gb = _gpu(a64, dev), _gpu(b64, dev)
+    s = ops.sum(ga).item()
+    d = ops.dot(ga, gb).item()
     hi, lo = ref.exact_sum_f32(a)
     dh, dl = ref.exact_dot_f32(a, b)
     assert abs(s - (hi + lo)) / (hi + lo) < 1e-13
