@@ -108,11 +108,18 @@ template <class P> NT2H_FD float exp_f(float a0) {
   r = (a0 <= -0x1.61814ap+6f) ? 0.0f : r;
   return (a0 != a0) ? a0 : r;
 }
-// exp of an argument known to be finite and <= 0 (call sites: -r*T, -x^2):
-// only the underflow edge needs a select.
+// exp of an argument known to be finite and <= 0 (call sites: -r*T, -x^2),
+// bit-identical to exp_core: k <= -127 (exactly when a0/ln2 rounds below
+// -126.5) gives 0 as the exponent-bits scale does, every k in [-126, 0]
+// scales exactly, so 2^k is applied with ldexp and one select.
 template <class P> NT2H_FD float exp_neg_f(float a0) {
-  float r = exp_core<P>(a0);
-  return (a0 <= -0x1.61814ap+6f) ? 0.0f : r;
+  float t = a0 * 0x1.715476p+0f;
+  float k = __builtin_rintf(t);
+  float x = P::nms(k, 0x1.63p-1f, a0);
+  x = P::nms(k, -0x1.bd0106p-13f, x);
+  float y = horner<P>(x, 0x1.6ef19ep-10f, 0x1.131b16p-7f, 0x1.5552aep-5f, 0x1.55534ap-3f, 0x1.0p-1f);
+  float c = 1.0f + P::madd(y, x * x, x);
+  return (t < -126.5f) ? 0.0f : __builtin_ldexpf(c, (int)k);
 }
 // exp2 / exp10 (f_expo_reduction.hpp:87-157)
 template <class P> NT2H_FD float exp2_f(float a0) {
