@@ -603,11 +603,34 @@ template <class P> NT2H_FD int reduce_d(double x, double* xr) {
   *xr = rh;
   return q;
 }
+// The float band (2^6 pi, 2^28] of the Exact reduction.  Up to 2^18 pi NT2
+// converts the float to double and runs its double medium reduction (five
+// products, nine differences; fdlibm's double medium for the last float,
+// 0x1.921fb6p+19); above it NT2 calls fdlibm's float Payne-Hanek, for which
+// this build gives the correctly rounded xr instead.  Two FMAs with pi/2 in
+// two double pieces (the first exact: k*P1 and x are multiples of 2^-52 and
+// their difference is below 2) give, after rounding to float, NT2's xr and
+// quadrant on every float up to 2^18 pi and the correctly rounded xr with
+// the nearest quadrant on every float above, up to 2^28 — checked
+// exhaustively (tests/cpp/trig_band_check.cpp; the first float that
+// differs lies near 2^29, where the rounding of x*2/pi picks the other
+// quadrant at |xr| = pi/4).  Outside the band the result is meaningless
+// (callers select).
+constexpr float kExactBandLoF = 0x1.921fb6p+7f;  // exclusive
+constexpr float kExactBandHiF = 0x1.0p+28f;      // inclusive
+NT2H_FD bool in_exact_band_f(float x) { return x > kExactBandLoF && x <= kExactBandHiF; }
+NT2H_FD int reduce_exact_f_band(float x, float* xr) {
+  const double xd = (double)x;
+  const double k = __builtin_rint(xd * 0x1.45f306dc9c883p-1);
+  double r = __builtin_fma(-k, 0x1.921fb54442d18p+0, xd);
+  r = __builtin_fma(-k, 0x1.1a62633145c07p-54, r);
+  *xr = (float)r;
+  return ((int32_t)k) & 3;
+}
 // Float argument reduction, Exact policy: NT2's scalar ladder for mode big_
 // (trig_reduction.hpp:94-247) — straight (<= pi/2), Cephes 3-part (<= 20 pi),
-// fdlibm-medium in float (<= 2^6 pi), the same medium in double after
-// conversion (<= 2^18 pi); beyond that NT2 calls fdlibm's float Payne-Hanek,
-// replaced here by the integer Payne-Hanek above (correctly rounded xr).
+// fdlibm-medium in float (<= 2^6 pi) — then the band above up to 2^28
+// (x <= 2^28 here).
 NT2H_FD int reduce_exact_f_near(float x, float* xr) {
   if (!(x > 0x1.921fb6p-1f)) { *xr = x; return 0; }
   if (!(x > 0x1.921fb6p+0f)) {
@@ -640,14 +663,10 @@ NT2H_FD int reduce_exact_f_near(float x, float* xr) {
     *xr = r - w;
     return ((int32_t)fn) & 3;
   }
-  // up to 2^18 pi (float): convert to double and run the double ladder
-  double xrd;
-  int n = reduce_d<Exact>((double)x, &xrd);
-  *xr = (float)xrd;
-  return n;
+  return reduce_exact_f_band(x, xr);
 }
-// Beyond 2^18 pi: the integer Payne-Hanek.
-constexpr float kExactFarF = 0x1.921fb6p+19f;
+// Beyond 2^28: the integer Payne-Hanek (correctly rounded xr).
+constexpr float kExactFarF = kExactBandHiF;
 NT2H_FD int reduce_exact_f_far(float x, float* xr) {
   uint32_t b = __float_as_uint(x);
   double rh, rl;
@@ -673,10 +692,12 @@ template <class P> NT2H_FD int reduce_f(float x, float* xr) {
 }
 // trig_base.hpp:131-161: cos / sin of a0 from the quadrant n and reduced
 // argument xr of |a0| (inf and NaN give NaN)
+// (both polynomials are evaluated and one selected: no divergent branch)
 template <class P> NT2H_FD float cos_from(float a0, int n, float xr) {
   bool bad = !(__builtin_fabsf(a0) <= 0x1.fffffep+127f);
   float z = xr * xr;
-  float r = (n & 1) ? sin_eval_f<P>(z, xr) : cos_eval_f<P>(z);
+  float se = sin_eval_f<P>(z, xr), ce = cos_eval_f<P>(z);
+  float r = (n & 1) ? se : ce;
   uint32_t sign = (uint32_t)(((n & 1) ^ ((n & 2) >> 1))) << 31;
   r = __uint_as_float(__float_as_uint(r) ^ sign);
   return bad ? __builtin_nanf("") : r;
@@ -684,7 +705,8 @@ template <class P> NT2H_FD float cos_from(float a0, int n, float xr) {
 template <class P> NT2H_FD float sin_from(float a0, int n, float xr) {
   bool bad = !(__builtin_fabsf(a0) <= 0x1.fffffep+127f);
   float z = xr * xr;
-  float r = (n & 1) ? cos_eval_f<P>(z) : sin_eval_f<P>(z, xr);
+  float se = sin_eval_f<P>(z, xr), ce = cos_eval_f<P>(z);
+  float r = (n & 1) ? ce : se;
   uint32_t sign = (__float_as_uint(a0) & 0x80000000u) ^ ((uint32_t)(n & 2) << 30);
   r = __uint_as_float(__float_as_uint(r) ^ sign);
   return bad ? __builtin_nanf("") : r;

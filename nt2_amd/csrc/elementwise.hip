@@ -146,14 +146,22 @@ template <class P> struct OpExp10 {
 };
 
 // EXACT float sin / cos over 16-byte vectors with the Payne-Hanek
-// arguments (|x| > 2^18 pi) compacted.  The integer Payne-Hanek costs more
+// arguments (|x| > 2^28) compacted.  The integer Payne-Hanek costs more
 // than the rest of the evaluation together, and run in place it occupies a
 // whole wave for each of the lane's four elements whenever any lane needs
-// it.  Instead each wave evaluates the ladder of NT2's reduction for its
-// other arguments, gathers the far ones (ballot + prefix count) into an LDS
+// it.  Instead each wave reduces the band (2^6 pi, 2^28] on every lane
+// without branching, runs the ladder of NT2's reduction only for lanes
+// holding smaller arguments, gathers the far ones (ballot + prefix count) into an LDS
 // list, reduces that list with all 64 lanes, and reads the results back; the
 // polynomial evaluation then runs on every element.  The arithmetic per
-// element is unchanged, so the results are those of sin_f / cos_f.
+// element is unchanged, so the results are those of sin_f / cos_f.  The
+// lists are per wave, so the waves synchronise on their own (no workgroup
+// barrier: a wave with many far arguments does not hold up the others).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 template <bool COS>
 __global__ __launch_bounds__(kBlock) void trig_exact_kernel(const float* __restrict__ x, float* __restrict__ y,
                                                             int64_t n) {
@@ -164,7 +172,6 @@ __global__ __launch_bounds__(kBlock) void trig_exact_kernel(const float* __restr
   const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));  // lanes under this one
   const int64_t nv = n / 4;
   const int64_t stride = (int64_t)gridDim.x * kBlock;
-  // every thread of the workgroup runs the same iterations (barriers inside)
   for (int64_t base = (int64_t)blockIdx.x * kBlock; base < nv; base += stride) {
     const int64_t i = base + threadIdx.x;
     const bool act = i < nv;
@@ -178,21 +185,23 @@ __global__ __launch_bounds__(kBlock) void trig_exact_kernel(const float* __restr
       const float ax = __builtin_fabsf(v[e]);
       const bool bad = !(ax <= 0x1.fffffep+127f);
       far[e] = !bad && ax > kExactFarF;
-      xr[e] = 0.0f;
-      q[e] = 0;
-      if (!far[e]) q[e] = reduce_exact_f_near(bad ? 0.0f : ax, &xr[e]);
+      // the band (all but 0.02% of arguments uniform in +-1e6) on every
+      // lane without a branch; NT2's ladder only where a lane holds a
+      // smaller argument
+      q[e] = reduce_exact_f_band(ax, &xr[e]);
+      if (!far[e] && !in_exact_band_f(ax)) q[e] = reduce_exact_f_near(bad ? 0.0f : ax, &xr[e]);
       const uint64_t m = __ballot(far[e]);
       slot[e] = total + __popcll(m & below);
       if (far[e]) phx[w][slot[e]] = ax;
       total += __popcll(m);
     }
-    __syncthreads();
+    wave_sync();
     for (int j = lane; j < total; j += 64) {
       float r;
       phq[w][j] = reduce_exact_f_far(phx[w][j], &r);
       phr[w][j] = r;
     }
-    __syncthreads();
+    wave_sync();
     f32x4 o;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
@@ -203,7 +212,7 @@ __global__ __launch_bounds__(kBlock) void trig_exact_kernel(const float* __restr
       o[e] = COS ? cos_from<Exact>(v[e], q[e], xr[e]) : sin_from<Exact>(v[e], q[e], xr[e]);
     }
     if (act) __builtin_nontemporal_store(o, reinterpret_cast<f32x4*>(y) + i);
-    __syncthreads();  // the lists are rewritten by the next iteration
+    wave_sync();  // the lists are rewritten by the next iteration
   }
   const int64_t t = 4 * nv + (int64_t)blockIdx.x * kBlock + threadIdx.x;  // tail [4 nv, n)
   if (t < n) y[t] = COS ? cos_f<Exact>(x[t]) : sin_f<Exact>(x[t]);
