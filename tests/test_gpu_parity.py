@@ -116,13 +116,15 @@ def test_map_f32_exact_bit_identical(dev, ref, fn):
 
 
 @pytest.mark.parametrize("fn", ["sin", "cos"])
-@pytest.mark.parametrize("lo,hi", [(-1e6, 1e6), (8.0e5, 8.4e5), (-3e7, 3e7), (-60, 60)])
+@pytest.mark.parametrize("lo,hi", [(-1e6, 1e6), (8.0e5, 8.4e5), (-3e7, 3e7), (-60, 60), (2.6e8, 2.8e8),
+                                   (-3e8, 3e8), (-1e10, 1e10)])
 def test_trig_exact_compacted_equals_scalar_path(dev, fn, lo, hi):
-    """EXACT float sin/cos over aligned tables gathers the Payne-Hanek
-    arguments of each wave into one list; the scalar path (an unaligned
-    table) evaluates every element in place.  Same arithmetic: bit-identical,
-    tails and specials included; mixes of near and far arguments per wave
-    from none to all."""
+    """EXACT float sin/cos over aligned tables reduces the band (2^6 pi, 2^28]
+    on every lane, runs NT2's ladder for smaller arguments and gathers the
+    Payne-Hanek arguments (beyond 2^28) of each wave into one list; the
+    scalar path (an unaligned table) evaluates every element in place.  Same
+    results: bit-identical, tails and specials included; mixes of band, small
+    and far arguments per wave from none to all."""
     import torch
 
     rng = np.random.default_rng(77)
