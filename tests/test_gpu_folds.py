@@ -254,6 +254,37 @@ def test_cumprod_powers_of_two_exact(dev, ref, shape):
         assert _bits_eq(got, want).all(), v
 
 
+@pytest.mark.parametrize("shape", [(1, (1 << 26) + 12, 1), (1, (1 << 26) + 5, 1), (1, 1 << 26, 2)])
+def test_long_run_scans(dev, ref, shape):
+    """Runs of 2^26 elements and more (three passes over thousands of
+    segments): 16-byte and scalar bodies, a partial last segment, two runs.
+    cummax/cummin bit-identical to NT2's recurrence, cumsum/cumprod within
+    the same bounds as the other shapes, and two calls give the same bits."""
+    lo, mid, hi = shape
+    n = lo * mid * hi
+    x = _specials_data(n, 31)
+    for op in ("cummax", "cummin"):
+        got = _scan(op, x, lo, mid, hi, dev)
+        assert _bits_eq(got, ref.scan_lmh(op, x, lo, mid, hi)).all(), op
+    rng = np.random.default_rng(32)
+    z = rng.standard_normal(n).astype(np.float32)
+    gz = _scan("cumsum", z, lo, mid, hi, dev)
+    # extended-precision reference: a float64 running sum over 2^26 terms
+    # drifts by ~1e-12 of the magnitudes itself
+    zl = z.astype(np.longdouble).reshape(hi, mid, lo)
+    wz = np.cumsum(zl, axis=1).reshape(-1)
+    mag = np.cumsum(np.abs(zl), axis=1).reshape(-1).astype(np.float64)
+    del zl
+    wz = wz.astype(np.float64)
+    assert np.all(np.abs(gz - wz) <= np.spacing(np.abs(wz).astype(np.float32)) + 1e-12 * mag)
+    assert _bits_eq(_scan("cumsum", z, lo, mid, hi, dev), gz).all()
+    del wz, mag
+    p = rng.uniform(1 - 1e-7, 1 + 1e-7, n).astype(np.float32)
+    gp = _scan("cumprod", p, lo, mid, hi, dev)
+    wp = np.cumprod(p.astype(np.float64).reshape(hi, mid, lo), axis=1).reshape(-1)
+    assert np.all(np.abs(gp - wp) <= 2 * np.spacing(wp.astype(np.float32)))
+
+
 def test_scan_unit_vectors(dev, ref):
     """modules/core/swar/unit/table/cumsum.cpp: reshape(_(1, 30), 3, 5, 2) along 1 and 2."""
     a = np.arange(1, 31, dtype=np.float32)
