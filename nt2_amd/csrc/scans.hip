@@ -29,9 +29,11 @@
 //  * column kernel (lo > 1, or short runs): one lane per (output, segment)
 //    running the recurrence — or, when lo % W == 0, per group of W adjacent
 //    outputs with 16-byte loads and stores; neighbouring lanes touch
-//    neighbouring addresses; very narrow tables (at most four such groups)
-//    use the tile kernel, whose waves read 16 or more adjacent rows per load
-//    and scan across them with shuffles;
+//    neighbouring addresses; narrow tables (at most 16 such groups) use the
+//    tile kernel, whose waves read 4 or more adjacent rows per load,
+//    transpose them through LDS so that each lane holds four consecutive
+//    rows, scan those in the lane and the lane totals across the wave with
+//    shuffles;
 //  * long runs split into segments: pass 1 computes each segment's total,
 //    pass 2 the exclusive prefix of the segment totals per run (one
 //    workgroup per run; one lane per output on the column paths), pass 3
@@ -393,17 +395,30 @@ __global__ __launch_bounds__(kBlock) void scan_colv_kernel(SGeo<T> g, int64_t se
   }
 }
 
-// Column kernel for very narrow tables (lo % W == 0 and at most four groups
-// of W outputs): a wave is R = 64 / P row phases of P lanes (P a power of
-// two).
-// Lane (r, p) holds the W outputs of group q = qb * P + p at row mb + r of
-// the wave's segment, so one load instruction of the wave reads R adjacent
-// rows (1 KiB of consecutive addresses when the groups cover the rows); the
-// R rows of a step are scanned across the phases with shuffles (lower rows
-// on the left), and the step total — the last phase's inclusive value — is
-// carried to the next step.  Four steps are loaded before the first is
-// scanned.  A wave past the last (segment, group block) leaves at once: no
-// workgroup barriers.  PASS as for the other column kernels.
+// Lanes of one wave exchanging data through LDS: make the writes visible
+// to the other lanes of the wave (no workgroup barrier).
+__device__ __forceinline__ void wave_sync_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Column kernel for narrow tables (lo % W == 0 and at most 16 groups of W
+// outputs): a wave is R = 64 / P row phases of P lanes (P a power of two).  A step covers K R rows (K = 4): the wave loads them with lane
+// (r, p) reading rows mb + r + R j (j < K) of group q = qb * P + p — one load
+// instruction reads R adjacent rows, 1 KiB of consecutive addresses when the
+// groups cover the rows — and transposes them through LDS (padded: one
+// 16-byte slot every four rows, so neither the writes nor the reads of four
+// consecutive rows per lane pile onto a few banks) so that lane (r, p) holds
+// the K consecutive rows mb + K r .. mb + K r + K - 1.  Each lane scans its K
+// rows in order, the lane totals are scanned across the phases with
+// shuffles (lower rows on the left), each row's result combines the
+// exclusive prefix of the lanes before with the lane's own prefix (the
+// results go back through LDS, so that the stores are as contiguous as the
+// loads), and the step total — the last phase's inclusive value — is
+// carried to the next step.  A wave past the last
+// (segment, group block) leaves at once: no workgroup barriers.  PASS as
+// for the other column kernels.
 template <int OP, class T, int PASS>
 __global__ __launch_bounds__(kBlock) void scan_tile_kernel(SGeo<T> g, int lgP, int64_t qblocks, int64_t segs,
                                                            int64_t seglen, SSumm* summ) {
@@ -412,7 +427,7 @@ __global__ __launch_bounds__(kBlock) void scan_tile_kernel(SGeo<T> g, int lgP, i
   using C = typename M::C;
   using VT = typename Vec16<T>::type;
   constexpr int W = Vec16<T>::n;
-  constexpr int U = 4;
+  constexpr int K = 4;
   const int P = 1 << lgP, R = 64 >> lgP;
   const int lane = threadIdx.x & 63;
   const int p = lane & (P - 1), r = lane >> lgP;
@@ -430,32 +445,72 @@ __global__ __launch_bounds__(kBlock) void scan_tile_kernel(SGeo<T> g, int lgP, i
   const int64_t step = g.lo / W;
   const int64_t m0 = s * seglen;
   const int64_t m1 = (m0 + seglen < g.mid) ? m0 + seglen : g.mid;
+  __shared__ VT tl[kWaves][K * 64 + K * 16];  // K R rows per group, P groups, one pad per four rows
+  VT* tw = tl[threadIdx.x >> 6];
+  const int gb = p * K * R;  // this group's rows in the wave's tile
   C c[W];
 #pragma unroll
   for (int k = 0; k < W; ++k) c[k] = (PASS == 3 && live) ? M::unpack(summ[s * nout + o + k]) : M::cident();
-  for (int64_t mb = m0; mb < m1; mb += U * R) {
-    VT v[U];
+  for (int64_t mb = m0; mb < m1; mb += (int64_t)K * R) {
+    const int64_t mr = mb + (int64_t)K * r;  // this lane's first row after the transposition
+    VT v[K];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int64_t m = mb + u * R + r;
-      v[u] = (live && m < m1) ? __builtin_nontemporal_load(px + m * step) : VT{};
+    for (int j = 0; j < K; ++j) {
+      const int64_t m = mb + r + (int64_t)R * j;
+      v[j] = (live && m < m1) ? __builtin_nontemporal_load(px + m * step) : VT{};
     }
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int64_t m = mb + u * R + r;
-      const bool ok = live && m < m1;
-      VT res;
+    for (int j = 0; j < K; ++j) {
+      const int t = gb + r + R * j;
+      tw[t + (t >> 2)] = v[j];
+    }
+    wave_sync_lds();
 #pragma unroll
-      for (int k = 0; k < W; ++k) {
-        V a = ok ? M::lift(v[u][k]) : M::ident();
-        for (int d = 1; d < R; d <<= 1) {
-          V lower = M::shfl_up(a, d * P);
-          if (r >= d) a = M::combs(lower, a);
-        }
-        res[k] = M::fin(c[k], a);
-        c[k] = M::cadd(c[k], M::shfl(a, last));
+    for (int j = 0; j < K; ++j) {
+      const int t = gb + K * r + j;
+      v[j] = tw[t + (t >> 2)];
+    }
+    wave_sync_lds();  // the tile is rewritten by the next step
+    VT res[K];
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+      V pre[K];
+      V a = M::ident();
+#pragma unroll
+      for (int j = 0; j < K; ++j) {
+        V xj = (live && mr + j < m1) ? M::lift(v[j][k]) : M::ident();
+        a = (j == 0) ? xj : M::combs(a, xj);
+        pre[j] = a;
       }
-      if (PASS != 1 && ok) __builtin_nontemporal_store(res, py + m * step);
+      V inc = a;
+      for (int d = 1; d < R; d <<= 1) {
+        V lower = M::shfl_up(inc, d * P);
+        if (r >= d) inc = M::combs(lower, inc);
+      }
+      if (PASS != 1) {
+        V ex = M::shfl_up(inc, P);
+        if (r == 0) ex = M::ident();
+#pragma unroll
+        for (int j = 0; j < K; ++j) res[j][k] = M::fin(c[k], M::combs(ex, pre[j]));
+      }
+      c[k] = M::cadd(c[k], M::tnorm(M::shfl(inc, last)));
+    }
+    if (PASS != 1) {
+      // back through LDS, so that each store instruction writes R adjacent rows
+#pragma unroll
+      for (int j = 0; j < K; ++j) {
+        const int t = gb + K * r + j;
+        tw[t + (t >> 2)] = res[j];
+      }
+      wave_sync_lds();
+#pragma unroll
+      for (int j = 0; j < K; ++j) {
+        const int t = gb + r + R * j;
+        const int64_t m = mb + r + (int64_t)R * j;
+        const VT out = tw[t + (t >> 2)];
+        if (live && m < m1) __builtin_nontemporal_store(out, py + m * step);
+      }
+      wave_sync_lds();
     }
   }
   if (PASS == 1 && r == 0 && live) {
@@ -592,9 +647,10 @@ int scan_launch(const T* x, T* y, int64_t lo, int64_t mid, int64_t hi, nt2h_stre
     return NT2H_OK;
   }
   const bool colv = (lo % W == 0) && aligned16(x, y);
-  // very narrow tables (at most four groups of W outputs, where the column
-  // kernel would run out of lanes): the row-phase tile kernel
-  const bool tile = colv && nout / W <= 4;
+  // narrow tables (at most 16 groups of W outputs: the column kernel runs
+  // out of lanes below that, and from 32 groups on it is faster): the
+  // row-phase tile kernel
+  const bool tile = colv && nout / W <= 16;
   int lgP = 0;
   int64_t qblocks = 0, segs, seglen, blocks;
   if (tile) {
