@@ -316,7 +316,8 @@ __global__ __launch_bounds__(kBlock) void scan_run_kernel(SGeo<T> g, int64_t seg
 // Column kernel: lane per (output, segment), the recurrence run in order.
 // PASS as above.
 template <int OP, class T, int PASS>
-__global__ __launch_bounds__(kBlock) void scan_col_kernel(SGeo<T> g, int64_t segs, int64_t seglen, SSumm* summ) {
+__global__ __launch_bounds__(kBlock) void scan_col_kernel(SGeo<T> g, int64_t segs, int64_t seglen, SSumm* summ,
+                                                         int64_t so, int64_t ss) {
   using M = typename SFam<OP, T>::type;
   const int64_t nout = g.lo * g.hi;
   const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -324,21 +325,22 @@ __global__ __launch_bounds__(kBlock) void scan_col_kernel(SGeo<T> g, int64_t seg
   const int64_t s = t / nout, o = t - s * nout;
   const int64_t m0 = s * seglen;
   const int64_t m1 = (m0 + seglen < g.mid) ? m0 + seglen : g.mid;
-  typename M::C c = (PASS == 3) ? M::unpack(summ[s * nout + o]) : M::cident();
+  typename M::C c = (PASS == 3) ? M::unpack(summ[o * so + s * ss]) : M::cident();
   typename M::V acc = M::ident();
   for (int64_t m = m0; m < m1; ++m) {
     const int64_t k = g.at(o, m);
     acc = M::comb(acc, M::lift(g.x[k]));
     if (PASS != 1) g.y[k] = M::fin(c, acc);
   }
-  if (PASS == 1) summ[s * nout + o] = M::pack(M::cadd(c, acc));
+  if (PASS == 1) summ[o * so + s * ss] = M::pack(M::cadd(c, acc));
 }
 
 // Column kernel, 16-byte form (lo % W == 0, x and y 16-byte aligned): one
 // lane per (group of W adjacent outputs, segment), one 16-byte load and
 // store per step along m, loads issued four steps ahead.
 template <int OP, class T, int PASS>
-__global__ __launch_bounds__(kBlock) void scan_colv_kernel(SGeo<T> g, int64_t segs, int64_t seglen, SSumm* summ) {
+__global__ __launch_bounds__(kBlock) void scan_colv_kernel(SGeo<T> g, int64_t segs, int64_t seglen, SSumm* summ,
+                                                          int64_t so, int64_t ss) {
   using M = typename SFam<OP, T>::type;
   using V = typename M::V;
   using C = typename M::C;
@@ -360,7 +362,7 @@ __global__ __launch_bounds__(kBlock) void scan_colv_kernel(SGeo<T> g, int64_t se
   V acc[W];
 #pragma unroll
   for (int k = 0; k < W; ++k) {
-    c[k] = (PASS == 3) ? M::unpack(summ[s * (g.lo * g.hi) + o + k]) : M::cident();
+    c[k] = (PASS == 3) ? M::unpack(summ[(o + k) * so + s * ss]) : M::cident();
     acc[k] = M::ident();
   }
   int64_t m = m0;
@@ -391,7 +393,7 @@ __global__ __launch_bounds__(kBlock) void scan_colv_kernel(SGeo<T> g, int64_t se
   }
   if (PASS == 1) {
 #pragma unroll
-    for (int k = 0; k < W; ++k) summ[s * (g.lo * g.hi) + o + k] = M::pack(M::cadd(c[k], acc[k]));
+    for (int k = 0; k < W; ++k) summ[(o + k) * so + s * ss] = M::pack(M::cadd(c[k], acc[k]));
   }
 }
 
@@ -421,7 +423,7 @@ __device__ __forceinline__ void wave_sync_lds() {
 // for the other column kernels.
 template <int OP, class T, int PASS>
 __global__ __launch_bounds__(kBlock) void scan_tile_kernel(SGeo<T> g, int lgP, int64_t qblocks, int64_t segs,
-                                                           int64_t seglen, SSumm* summ) {
+                                                           int64_t seglen, SSumm* summ, int64_t so, int64_t ss) {
   using M = typename SFam<OP, T>::type;
   using V = typename M::V;
   using C = typename M::C;
@@ -450,7 +452,7 @@ __global__ __launch_bounds__(kBlock) void scan_tile_kernel(SGeo<T> g, int lgP, i
   const int gb = p * K * R;  // this group's rows in the wave's tile
   C c[W];
 #pragma unroll
-  for (int k = 0; k < W; ++k) c[k] = (PASS == 3 && live) ? M::unpack(summ[s * nout + o + k]) : M::cident();
+  for (int k = 0; k < W; ++k) c[k] = (PASS == 3 && live) ? M::unpack(summ[(o + k) * so + s * ss]) : M::cident();
   for (int64_t mb = m0; mb < m1; mb += (int64_t)K * R) {
     const int64_t mr = mb + (int64_t)K * r;  // this lane's first row after the transposition
     VT v[K];
@@ -515,7 +517,7 @@ __global__ __launch_bounds__(kBlock) void scan_tile_kernel(SGeo<T> g, int lgP, i
   }
   if (PASS == 1 && r == 0 && live) {
 #pragma unroll
-    for (int k = 0; k < W; ++k) summ[s * nout + o + k] = M::pack(c[k]);
+    for (int k = 0; k < W; ++k) summ[(o + k) * so + s * ss] = M::pack(c[k]);
   }
 }
 
@@ -653,6 +655,7 @@ int scan_launch(const T* x, T* y, int64_t lo, int64_t mid, int64_t hi, nt2h_stre
   const bool tile = colv && nout / W <= 16;
   int lgP = 0;
   int64_t qblocks = 0, segs, seglen, blocks;
+  int64_t so = 1, ss = 0;  // segment total of (o, s) at summ[o * so + s * ss], set below
   if (tile) {
     // about 16384 waves (the shuffle scan across the phases makes the waves
     // latency-bound: measured fastest between 8192 and 32768), at least
@@ -686,11 +689,11 @@ int scan_launch(const T* x, T* y, int64_t lo, int64_t mid, int64_t hi, nt2h_stre
 #define NT2H_SCAN_COL(PASS)                                                                                      \
   do {                                                                                                            \
     if (tile) hipLaunchKernelGGL((scan_tile_kernel<OP, T, PASS>), dim3((unsigned)blocks), dim3(kBlock), 0, s, g, \
-                                 lgP, qblocks, segs, seglen, summ);                                               \
+                                 lgP, qblocks, segs, seglen, summ, so, ss);                                       \
     else if (colv) hipLaunchKernelGGL((scan_colv_kernel<OP, T, PASS>), dim3((unsigned)blocks), dim3(kBlock), 0,  \
-                                      s, g, segs, seglen, summ);                                                  \
+                                      s, g, segs, seglen, summ, so, ss);                                          \
     else hipLaunchKernelGGL((scan_col_kernel<OP, T, PASS>), dim3((unsigned)blocks), dim3(kBlock), 0, s, g, segs, \
-                            seglen, summ);                                                                        \
+                            seglen, summ, so, ss);                                                                \
     NT2H_HIP(hipGetLastError());                                                                                  \
   } while (0)
   if (segs == 1) {
@@ -698,13 +701,19 @@ int scan_launch(const T* x, T* y, int64_t lo, int64_t mid, int64_t hi, nt2h_stre
     return NT2H_OK;
   }
   NT2H_TRY(get_scratch(st, (size_t)(nout * segs) * sizeof(SSumm), (void**)&summ));
+  // segment totals segment-major (s * nout + o) for one lane per output,
+  // output-major (o * segs + s) for one workgroup per output, so that the
+  // prefix pass reads them contiguously
+  const bool lane_per_output = nout >= 8192 || segs <= 16;
+  so = lane_per_output ? 1 : segs;
+  ss = lane_per_output ? nout : 1;
   NT2H_SCAN_COL(1);
-  if (nout >= 8192 || segs <= 16)  // enough outputs for one lane each
+  if (lane_per_output)
     hipLaunchKernelGGL((scan_col_prefix_kernel<OP, T>), dim3((unsigned)((nout + kBlock - 1) / kBlock)), dim3(kBlock),
                        0, s, nout, segs, summ);
   else
     hipLaunchKernelGGL((scan_prefix_kernel<OP, T>), dim3((unsigned)nout), dim3(kPrefixBlock), 0, s, segs, summ,
-                       (int64_t)1, nout);
+                       segs, (int64_t)1);
   NT2H_HIP(hipGetLastError());
   NT2H_SCAN_COL(3);
 #undef NT2H_SCAN_COL
