@@ -53,6 +53,11 @@ inline unsigned stream_grid(int64_t n, int vec, int64_t max_blocks = 65536) {
 
 int get_workspace(nt2h_stream_t s, Workspace* ws);
 int get_scratch(nt2h_stream_t s, size_t bytes, void** out);
+// Per (device, stream) state of the single-pass look-back scans: at least
+// `bytes`, zero when first returned and then owned by those kernels (never
+// shared with the scratch above).  NT2H_ERR_CAPTURE if it would have to
+// grow while the stream is capturing.
+int get_lookback(nt2h_stream_t s, size_t bytes, void** out);
 // Deliver nv doubles computed into dev (device memory) to out (device or
 // host memory); for host memory the stream is synchronised.
 int deliver(nt2h_stream_t s, double* out, const double* dev, int nv, bool out_is_host);

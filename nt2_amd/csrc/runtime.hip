@@ -103,6 +103,40 @@ int get_scratch(nt2h_stream_t s, size_t bytes, void** out) {
   return NT2H_OK;
 }
 
+namespace {
+struct Lookback {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+std::mutex g_lb_mutex;
+std::map<std::pair<int, hipStream_t>, Lookback> g_lb;
+}  // namespace
+
+int get_lookback(nt2h_stream_t s, size_t bytes, void** out) {
+  int dev = 0;
+  NT2H_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(g_lb_mutex);
+  Lookback& lb = g_lb[std::make_pair(dev, as_hip(s))];
+  if (lb.bytes < bytes) {
+    if (capturing(s)) return NT2H_ERR_CAPTURE;
+    if (lb.p) {
+      NT2H_HIP(hipStreamSynchronize(as_hip(s)));
+      NT2H_HIP(hipFree(lb.p));
+      lb.p = nullptr;
+      lb.bytes = 0;
+    }
+    size_t want = (size_t)1 << 20;
+    while (want < bytes) want <<= 1;
+    NT2H_HIP(hipMalloc(&lb.p, want));
+    // zeroed once, synchronously: the kernels keep it consistent (epoch
+    // tags on the tile states, counters reset by the last workgroup)
+    NT2H_HIP(hipMemset(lb.p, 0, want));
+    lb.bytes = want;
+  }
+  *out = lb.p;
+  return NT2H_OK;
+}
+
 int is_host_pointer(const void* p, bool* host, nt2h_stream_t s) {
   // Under stream capture the attribute query is refused (and would
   // invalidate the capture), and a host result would need a stream

@@ -37,7 +37,11 @@
 //  * long runs split into segments: pass 1 computes each segment's total,
 //    pass 2 the exclusive prefix of the segment totals per run (one
 //    workgroup per run; one lane per output on the column paths), pass 3
-//    rescans each segment from its prefix.
+//    rescans each segment from its prefix;
+//  * runs of 2^14 to 2^19 floats (16-byte vectors): one launch with
+//    decoupled look-back between tiles, the prefix of each tile folded in
+//    the order of the serial recurrence so that the results are
+//    reproducible.
 #include <hip/hip_runtime.h>
 
 #include "nt2h_common.hpp"
@@ -606,6 +610,193 @@ __global__ __launch_bounds__(kPrefixBlock) void scan_prefix_kernel(int64_t segs,
   exclusive_range<M>(M::ccomb(wpre, ex), base, s0, s1, sstride);
 }
 
+// Single-pass scan of long runs (lo == 1, 16-byte vectors) with decoupled
+// look-back.  Workgroups take tiles of kLbSub x kBlock x W elements in
+// ticket order; each scans its tile in registers (as run_segment does, the
+// tile aggregate A falling out), publishes A, and finds the exclusive
+// prefix P of its tile from the published state of the tiles before it:
+// wave 0 reads the states of up to 64 predecessors at once, takes the
+// nearest one that has published its inclusive value I_j (every tile in
+// between having published its aggregate) and folds
+//   P = (((I_j + A_{j+1}) + A_{j+2}) + ... ) + A_{i-1}
+// left to right — the grouping of the serial recurrence I_k = I_{k-1} + A_k,
+// so the result does not depend on which j the timing offered: the output
+// bits are the same on every run.  It then publishes I_i = P + A_i and
+// writes its tile.  Every tile reads and writes its input and output once,
+// in one launch (the three passes take three).
+//
+// Publication: the values are written with agent-scope atomic stores
+// (through to the point of coherence of all XCDs), completed (s_waitcnt),
+// then the tile's state word — (epoch << 32) | 1 for "aggregate", | 2 for
+// "inclusive" — with another; readers load the state word and then the
+// values with agent-scope atomic loads.  The epoch (the number of previous
+// launches on this state) makes the state of earlier calls read as "not
+// yet"; the last workgroup to finish resets the ticket and done counters
+// and advances the epoch, so no memset is ever needed (graph-safe).
+// Forward progress: a tile waits only on tiles with smaller tickets, which
+// are already running and publish their aggregate before waiting on
+// anything.  Every wait is nevertheless bounded (spin_cap polls with a
+// sleep between them): past it the workgroup records an error in ctrl[3]
+// and finishes, so the kernel always ends.
+constexpr int kLbSub = 4;  // sub-tiles per look-back tile
+constexpr unsigned kLbSpinCap = 1u << 20;
+// The inclusive values advance by at most 64 tiles per memory round trip
+// (one wave reads 64 states), so a run's last tile finishes about
+// tpr / 64 round trips after its first: measured faster than the three
+// passes (three launches) up to 128 tiles per run — 2^19 floats, 2.4x at
+// 2^14, 1.45x at 2^18 — and slower beyond (8x at 2^28).
+constexpr int64_t kLbMaxTiles = 128;
+__device__ __forceinline__ void lb_put(SSumm* p, const SSumm& v) {
+  publish(&p->w[0], v.w[0]);
+  publish(&p->w[1], v.w[1]);
+}
+__device__ __forceinline__ SSumm lb_get(const SSumm* p) { return SSumm{{fetch(&p->w[0]), fetch(&p->w[1])}}; }
+__device__ __forceinline__ SSumm shfl_summ(const SSumm& v, int src) {
+  return SSumm{{__shfl(v.w[0], src, 64), __shfl(v.w[1], src, 64)}};
+}
+__device__ __forceinline__ void lb_state(unsigned long long* f, unsigned long long v) {
+  __hip_atomic_store(f, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned long long lb_state_get(const unsigned long long* f) {
+  return __hip_atomic_load(const_cast<unsigned long long*>(f), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <int OP, class T>
+__global__ __launch_bounds__(kBlock) void scan_lookback_kernel(SGeo<T> g, int64_t tpr, int64_t ntiles,
+                                                               unsigned* ctrl, unsigned long long* state,
+                                                               SSumm* agg, SSumm* incl, unsigned spin_cap) {
+  using M = typename SFam<OP, T>::type;
+  using V = typename M::V;
+  using C = typename M::C;
+  using VT = typename Vec16<T>::type;
+  constexpr int W = Vec16<T>::n;
+  constexpr int64_t kSub = (int64_t)kBlock * W;
+  __shared__ V lds[kLbSub][kWaves];
+  __shared__ unsigned sh_t[2];
+  __shared__ SSumm sh_p;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (threadIdx.x == 0) {
+    sh_t[0] = __hip_atomic_fetch_add(&ctrl[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    sh_t[1] = __hip_atomic_load(&ctrl[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();
+  const int64_t t = sh_t[0];
+  const unsigned epoch = sh_t[1];
+  const int64_t o = t / tpr, i = t - o * tpr;  // run, tile within the run
+  const int64_t m0 = i * kLbSub * kSub;
+  const int64_t m1 = (m0 + kLbSub * kSub < g.mid) ? m0 + kLbSub * kSub : g.mid;
+  const T* run = g.x + o * g.mid;
+  T* out = g.y + o * g.mid;
+  // the whole tile in registers (mid % W == 0: a vector is all in or all out)
+  VT e[kLbSub];
+#pragma unroll
+  for (int u = 0; u < kLbSub; ++u) {
+    const int64_t m = m0 + u * kSub + W * (int64_t)threadIdx.x;
+    e[u] = (m < m1) ? __builtin_nontemporal_load(reinterpret_cast<const VT*>(run + m)) : VT{};
+  }
+  V p[kLbSub][W], inc[kLbSub];
+#pragma unroll
+  for (int u = 0; u < kLbSub; ++u) {
+    const int64_t m = m0 + u * kSub + W * (int64_t)threadIdx.x;
+    V acc = M::ident();
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+      V xk = (m < m1) ? M::lift(e[u][k]) : M::ident();
+      acc = (k == 0) ? xk : M::combs(acc, xk);
+      p[u][k] = acc;
+    }
+    inc[u] = wave_inscan<M>(acc);
+    if (lane == 63) lds[u][wave] = M::tnorm(inc[u]);
+  }
+  __syncthreads();
+  C L[kLbSub + 1];  // local carries: the tile's own prefix before each sub-tile
+  V wpre[kLbSub];
+  L[0] = M::cident();
+#pragma unroll
+  for (int u = 0; u < kLbSub; ++u) {
+    V tot = M::ident();
+    wpre[u] = M::ident();
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+      V tw = lds[u][w];
+      if (w < wave) wpre[u] = M::combs(wpre[u], tw);
+      tot = M::combs(tot, tw);
+    }
+    L[u + 1] = M::cadd(L[u], tot);
+  }
+  if (wave == 0) {
+    const unsigned long long tag = (unsigned long long)epoch << 32;
+    C P = M::cident();
+    if (i == 0) {
+      if (lane == 0) {
+        lb_put(&incl[t], M::pack(L[kLbSub]));
+        __builtin_amdgcn_s_waitcnt(0);
+        lb_state(&state[t], tag | 2ull);
+      }
+    } else {
+      if (lane == 0) {
+        lb_put(&agg[t], M::pack(L[kLbSub]));
+        __builtin_amdgcn_s_waitcnt(0);
+        lb_state(&state[t], tag | 1ull);
+      }
+      int jl = -1;  // lane (= distance - 1) of the nearest inclusive predecessor
+      for (unsigned spins = 0;; ++spins) {
+        const bool inrun = i - 1 - lane >= 0;
+        const unsigned long long f = inrun ? lb_state_get(&state[t - 1 - lane]) : 0ull;
+        const bool ready = inrun && (f >> 32) == epoch && (f & 3ull) != 0;
+        const uint64_t mi = __ballot(ready && (f & 3ull) == 2ull), mr = __ballot(ready);
+        if (mi) {
+          const int j = __builtin_ctzll(mi);
+          const uint64_t need = (j == 0) ? 0ull : (~0ull >> (64 - j));  // lanes 0..j-1
+          if ((mr & need) == need) {
+            jl = j;
+            break;
+          }
+        }
+        if (spins >= spin_cap) {
+          if (lane == 0) __hip_atomic_store(&ctrl[3], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+      }
+      if (jl >= 0) {
+        SSumm v{{0.0, 0.0}};
+        if (lane == jl) v = lb_get(&incl[t - 1 - lane]);
+        else if (lane < jl) v = lb_get(&agg[t - 1 - lane]);
+        P = M::unpack(shfl_summ(v, jl));
+        for (int l = jl - 1; l >= 0; --l) P = M::ccomb(P, M::unpack(shfl_summ(v, l)));
+      }
+      if (lane == 0) {
+        lb_put(&incl[t], M::pack(M::ccomb(P, L[kLbSub])));
+        __builtin_amdgcn_s_waitcnt(0);
+        lb_state(&state[t], tag | 2ull);
+      }
+    }
+    if (lane == 0) sh_p = M::pack(P);
+  }
+  __syncthreads();
+  const C P = M::unpack(sh_p);
+#pragma unroll
+  for (int u = 0; u < kLbSub; ++u) {
+    const int64_t m = m0 + u * kSub + W * (int64_t)threadIdx.x;
+    V ex = M::shfl_up(inc[u], 1);
+    if (lane == 0) ex = M::ident();
+    ex = M::combs(wpre[u], ex);
+    const C carry = M::ccomb(P, L[u]);
+    VT r;
+#pragma unroll
+    for (int k = 0; k < W; ++k) r[k] = M::fin(carry, M::combs(ex, p[u][k]));
+    if (m < m1) __builtin_nontemporal_store(r, reinterpret_cast<VT*>(out + m));
+  }
+  if (threadIdx.x == 0) {
+    const unsigned d = __hip_atomic_fetch_add(&ctrl[1], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if ((int64_t)d == ntiles - 1) {  // every workgroup has taken its ticket and finished
+      __hip_atomic_store(&ctrl[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&ctrl[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&ctrl[2], epoch + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
 template <int OP, class T>
 int scan_launch(const T* x, T* y, int64_t lo, int64_t mid, int64_t hi, nt2h_stream_t st) {
   constexpr int W = Vec16<T>::n;
@@ -637,6 +828,25 @@ int scan_launch(const T* x, T* y, int64_t lo, int64_t mid, int64_t hi, nt2h_stre
   } while (0)
     if (segs == 1) {
       NT2H_SCAN_RUN(0);
+      return NT2H_OK;
+    }
+    // runs of up to kLbMaxTiles look-back tiles with 16-byte vectors: one
+    // pass with decoupled look-back (longer runs: three passes, see
+    // kLbMaxTiles)
+    const int64_t lbtile = (int64_t)kLbSub * kBlock * W;
+    const int64_t tpr = (mid + lbtile - 1) / lbtile;
+    if (vec && tpr <= kLbMaxTiles && nout * tpr <= ((int64_t)1 << 22)) {
+      const int64_t ntiles = nout * tpr;
+      const size_t need = 256 + (size_t)ntiles * (sizeof(unsigned long long) + 2 * sizeof(SSumm));
+      void* base = nullptr;
+      NT2H_TRY(get_lookback(st, need, &base));
+      unsigned* ctrl = reinterpret_cast<unsigned*>(base);
+      unsigned long long* lbst = reinterpret_cast<unsigned long long*>(reinterpret_cast<char*>(base) + 256);
+      SSumm* agg = reinterpret_cast<SSumm*>(lbst + ntiles);
+      SSumm* incl = agg + ntiles;
+      hipLaunchKernelGGL((scan_lookback_kernel<OP, T>), dim3((unsigned)ntiles), dim3(kBlock), 0, s, g, tpr, ntiles,
+                         ctrl, lbst, agg, incl, kLbSpinCap);
+      NT2H_HIP(hipGetLastError());
       return NT2H_OK;
     }
     NT2H_TRY(get_scratch(st, (size_t)blocks * sizeof(SSumm), (void**)&summ));

@@ -285,6 +285,34 @@ def test_long_run_scans(dev, ref, shape):
     assert np.all(np.abs(gp - wp) <= 2 * np.spacing(wp.astype(np.float32)))
 
 
+@pytest.mark.parametrize("shape", [(1, 1 << 19, 1), (1, (1 << 16) + 4, 3), (1, 20000, 1), (1, 16384, 2)])
+def test_lookback_scans(dev, ref, shape):
+    """Runs of 2^14 to 2^19 floats are scanned in one launch, tiles finding
+    their prefix from the published state of the tiles before them (a
+    partial last tile, several runs).  cummax/cummin bit-identical to NT2's
+    recurrence, cumsum/cumprod within the bounds of the other paths, and the
+    bits do not depend on the timing: three calls agree."""
+    lo, mid, hi = shape
+    n = lo * mid * hi
+    x = _specials_data(n, 41)
+    for op in ("cummax", "cummin"):
+        got = _scan(op, x, lo, mid, hi, dev)
+        assert _bits_eq(got, ref.scan_lmh(op, x, lo, mid, hi)).all(), op
+    rng = np.random.default_rng(42)
+    z = rng.standard_normal(n).astype(np.float32)
+    gz = _scan("cumsum", z, lo, mid, hi, dev)
+    wz = np.cumsum(z.astype(np.float64).reshape(hi, mid, lo), axis=1).reshape(-1)
+    mag = np.cumsum(np.abs(z.astype(np.float64)).reshape(hi, mid, lo), axis=1).reshape(-1)
+    assert np.all(np.abs(gz - wz) <= np.spacing(np.abs(wz).astype(np.float32)) + 1e-12 * mag)
+    p = rng.uniform(0.999, 1.001, n).astype(np.float32)
+    gp = _scan("cumprod", p, lo, mid, hi, dev)
+    wp = np.cumprod(p.astype(np.float64).reshape(hi, mid, lo), axis=1).reshape(-1)
+    assert np.all(np.abs(gp - wp) <= 2 * np.spacing(wp.astype(np.float32)))
+    for _ in range(2):
+        assert _bits_eq(_scan("cumsum", z, lo, mid, hi, dev), gz).all()
+        assert _bits_eq(_scan("cumprod", p, lo, mid, hi, dev), gp).all()
+
+
 def test_scan_unit_vectors(dev, ref):
     """modules/core/swar/unit/table/cumsum.cpp: reshape(_(1, 30), 3, 5, 2) along 1 and 2."""
     a = np.arange(1, 31, dtype=np.float32)

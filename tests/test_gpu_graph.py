@@ -139,3 +139,46 @@ def test_graph_fold_and_scan_replays(dev):
         g.replay()
         torch.cuda.synchronize()
         assert torch.equal(f, wf) and torch.equal(y, wy)
+
+
+def test_graph_lookback_scan_replays(dev):
+    """The single-launch look-back scan keeps per-stream tile states tagged
+    with a launch epoch that the kernel itself advances: replays, with eager
+    calls of the same scan on the same stream between them, give the eager
+    bits every time."""
+    import torch
+
+    lo, mid, hi = 1, 1 << 18, 2
+    x = torch.from_numpy(np.random.default_rng(5).standard_normal(lo * mid * hi).astype(np.float32)).to(dev)
+    y = torch.empty_like(x)
+    m = torch.empty_like(x)
+
+    def step():
+        nt2_amd.call("nt2h_scan_f32", nt2_amd.SCAN_OPS["cumsum"], x.data_ptr(), lo, mid, hi, y.data_ptr(), _st())
+        nt2_amd.call("nt2h_scan_f32", nt2_amd.SCAN_OPS["cummax"], x.data_ptr(), lo, mid, hi, m.data_ptr(), _st())
+
+    step()
+    torch.cuda.synchronize()
+    wy, wm = y.clone(), m.clone()
+    cs = torch.cuda.Stream()  # the stream whose state the graph's kernels use
+    cs.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(cs):
+        step()
+    cs.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=cs):
+        step()
+    torch.cuda.synchronize()
+    for _ in range(4):
+        y.zero_()
+        m.zero_()
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(y, wy) and torch.equal(m, wm)
+        y.zero_()
+        torch.cuda.synchronize()
+        with torch.cuda.stream(cs):  # eager launches on the same state between replays
+            step()
+        cs.synchronize()
+        assert torch.equal(y, wy) and torch.equal(m, wm)
