@@ -454,6 +454,45 @@ __global__ __launch_bounds__(kBlock) void scan_tile_kernel(SGeo<T> g, int lgP, i
   __shared__ VT tl[kWaves][K * 64 + K * 16];  // K R rows per group, P groups, one pad per four rows
   VT* tw = tl[threadIdx.x >> 6];
   const int gb = p * K * R;  // this group's rows in the wave's tile
+  if (PASS == 1 && M::kComm) {
+    // segment totals of sum and product: each lane folds its rows in load
+    // order (no transposition, no scan), the phases are combined once at
+    // the end in a fixed tree
+    V acc[W];
+#pragma unroll
+    for (int k = 0; k < W; ++k) acc[k] = M::ident();
+    for (int64_t mb = m0; mb < m1; mb += (int64_t)K * R) {
+      VT v[K];
+#pragma unroll
+      for (int j = 0; j < K; ++j) {
+        const int64_t m = mb + r + (int64_t)R * j;
+        v[j] = (live && m < m1) ? __builtin_nontemporal_load(px + m * step) : VT{};
+      }
+#pragma unroll
+      for (int k = 0; k < W; ++k) {
+        V a = M::ident();
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+          const bool ok = live && mb + r + (int64_t)R * j < m1;
+          a = M::combs(a, ok ? M::lift(v[j][k]) : M::ident());
+        }
+        acc[k] = M::comb(acc[k], a);
+      }
+    }
+    for (int d = R >> 1; d >= 1; d >>= 1) {
+      const int src = (r + d < R) ? lane + d * P : lane;
+#pragma unroll
+      for (int k = 0; k < W; ++k) {
+        V other = M::shfl(acc[k], src);
+        if (r < d) acc[k] = M::comb(acc[k], other);
+      }
+    }
+    if (r == 0 && live) {
+#pragma unroll
+      for (int k = 0; k < W; ++k) summ[(o + k) * so + s * ss] = M::pack(M::cadd(M::cident(), M::tnorm(acc[k])));
+    }
+    return;
+  }
   C c[W];
 #pragma unroll
   for (int k = 0; k < W; ++k) c[k] = (PASS == 3 && live) ? M::unpack(summ[(o + k) * so + s * ss]) : M::cident();
