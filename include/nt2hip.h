@@ -6,8 +6,9 @@
  * inner_fold_ to a CUDA site (modules/arch/cuda/include/nt2/sdk/cuda/
  * cuda.hpp:22-41 registers none), a caller binds these entry points.  Every
  * entry point evaluates one fused loop nest in one pass over its tables (one
- * kernel launch; long folds and scans add small launches that combine
- * segment partials).
+ * kernel launch; long folds add a small launch that combines segment
+ * partials, and scans of long runs or of many rows take three passes:
+ * segment totals, their prefixes, the rescan).
  *
  * Conventions
  *  - Plain C types only.  Pointers are caller-owned device pointers unless a
@@ -235,7 +236,8 @@ int nt2h_bs_fwd_adj_f64(const double* S, const double* K, const double* T, const
  * dz_ds_j = gz_j*sum e c X^2/s^3, dz_dk_j = -gz_j*sum e sin(.) X,
  * dz_dp_j = -gz_j*sum e sin(.).  Gradient pointers may all be NULL (forward
  * only).  X is rows x cols, column-major, leading dimension ld >= rows;
- * per-column compensated float accumulation. */
+ * per-lane FP64 accumulation of float pairwise sums of four elements, then a
+ * fixed-order FP64 combine per column. */
 int nt2h_colfold_gc_f32(const float* X, int64_t rows, int64_t cols, int64_t ld, const float* s_,
                         const float* k_, const float* p_, const float* gz, float* z, float* dz_ds,
                         float* dz_dk, float* dz_dp, nt2h_stream_t s);
