@@ -38,10 +38,10 @@
 //    pass 2 the exclusive prefix of the segment totals per run (one
 //    workgroup per run; one lane per output on the column paths), pass 3
 //    rescans each segment from its prefix;
-//  * runs of 2^14 to 2^19 floats (16-byte vectors): one launch with
-//    decoupled look-back between tiles, the prefix of each tile folded in
-//    the order of the serial recurrence so that the results are
-//    reproducible.
+//  * runs of 2^14 to 2^19 floats, 2^21 floats in all (16-byte vectors):
+//    one launch with decoupled look-back between tiles, the prefix of each
+//    tile folded in the order of the serial recurrence so that the results
+//    are reproducible.
 #include <hip/hip_runtime.h>
 
 #include "nt2h_common.hpp"
@@ -670,8 +670,11 @@ __global__ __launch_bounds__(kPrefixBlock) void scan_prefix_kernel(int64_t segs,
 // "inclusive" — with another; readers load the state word and then the
 // values with agent-scope atomic loads.  The epoch (the number of previous
 // launches on this state) makes the state of earlier calls read as "not
-// yet"; the last workgroup to finish resets the ticket and done counters
-// and advances the epoch, so no memset is ever needed (graph-safe).
+// yet"; the workgroup holding the last ticket resets the ticket counter and
+// advances the epoch, so no memset is ever needed (graph-safe).  The ticket
+// is the kernel's only atomic on a shared address: such atomics are served
+// one at a time (~35 ns each measured), which is what bounds this kernel to
+// small tile counts.
 // Forward progress: a tile waits only on tiles with smaller tickets, which
 // are already running and publish their aggregate before waiting on
 // anything.  Every wait is nevertheless bounded (spin_cap polls with a
@@ -679,12 +682,14 @@ __global__ __launch_bounds__(kPrefixBlock) void scan_prefix_kernel(int64_t segs,
 // and finishes, so the kernel always ends.
 constexpr int kLbSub = 4;  // sub-tiles per look-back tile
 constexpr unsigned kLbSpinCap = 1u << 20;
-// The inclusive values advance by at most 64 tiles per memory round trip
-// (one wave reads 64 states), so a run's last tile finishes about
-// tpr / 64 round trips after its first: measured faster than the three
-// passes (three launches) up to 128 tiles per run — 2^19 floats, 2.4x at
-// 2^14, 1.45x at 2^18 — and slower beyond (8x at 2^28).
-constexpr int64_t kLbMaxTiles = 128;
+// Two limits, both measured against the three passes (three launches):
+// the tickets cost ~35 ns each, one after the other (16384 tiles: 887 us
+// against 154), and the inclusive values of a run advance by at most 64
+// tiles per memory round trip.  Up to 512 tiles in all and 128 per run the
+// single launch wins (2^14 floats: 18.5 -> 7.2 us, 2^18: 19.1 -> 13.0 us,
+// 16 runs of 2^16: 22.8 -> 12.4 us); beyond, three passes.
+constexpr int64_t kLbMaxTiles = 128;  // per run
+constexpr int64_t kLbMaxTotal = 512;
 __device__ __forceinline__ void lb_put(SSumm* p, const SSumm& v) {
   publish(&p->w[0], v.w[0]);
   publish(&p->w[1], v.w[1]);
@@ -714,8 +719,12 @@ __global__ __launch_bounds__(kBlock) void scan_lookback_kernel(SGeo<T> g, int64_
   __shared__ SSumm sh_p;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   if (threadIdx.x == 0) {
-    sh_t[0] = __hip_atomic_fetch_add(&ctrl[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // the epoch is read before the ticket is taken (and the load completed):
+    // the tile holding the last ticket advances it only after every ticket
+    // is gone, so every workgroup of this launch sees the same epoch
     sh_t[1] = __hip_atomic_load(&ctrl[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_s_waitcnt(0);
+    sh_t[0] = __hip_atomic_fetch_add(&ctrl[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   __syncthreads();
   const int64_t t = sh_t[0];
@@ -826,13 +835,9 @@ __global__ __launch_bounds__(kBlock) void scan_lookback_kernel(SGeo<T> g, int64_
     for (int k = 0; k < W; ++k) r[k] = M::fin(carry, M::combs(ex, p[u][k]));
     if (m < m1) __builtin_nontemporal_store(r, reinterpret_cast<VT*>(out + m));
   }
-  if (threadIdx.x == 0) {
-    const unsigned d = __hip_atomic_fetch_add(&ctrl[1], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if ((int64_t)d == ntiles - 1) {  // every workgroup has taken its ticket and finished
-      __hip_atomic_store(&ctrl[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(&ctrl[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(&ctrl[2], epoch + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
+  if (threadIdx.x == 0 && t == ntiles - 1) {  // every ticket is taken, every epoch read
+    __hip_atomic_store(&ctrl[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&ctrl[2], epoch + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
@@ -869,12 +874,12 @@ int scan_launch(const T* x, T* y, int64_t lo, int64_t mid, int64_t hi, nt2h_stre
       NT2H_SCAN_RUN(0);
       return NT2H_OK;
     }
-    // runs of up to kLbMaxTiles look-back tiles with 16-byte vectors: one
-    // pass with decoupled look-back (longer runs: three passes, see
-    // kLbMaxTiles)
+    // up to kLbMaxTiles look-back tiles per run and kLbMaxTotal in all, with
+    // 16-byte vectors: one pass with decoupled look-back (beyond: three
+    // passes, see kLbMaxTiles)
     const int64_t lbtile = (int64_t)kLbSub * kBlock * W;
     const int64_t tpr = (mid + lbtile - 1) / lbtile;
-    if (vec && tpr <= kLbMaxTiles && nout * tpr <= ((int64_t)1 << 22)) {
+    if (vec && tpr <= kLbMaxTiles && nout * tpr <= kLbMaxTotal) {
       const int64_t ntiles = nout * tpr;
       const size_t need = 256 + (size_t)ntiles * (sizeof(unsigned long long) + 2 * sizeof(SSumm));
       void* base = nullptr;
