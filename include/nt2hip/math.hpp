@@ -607,7 +607,7 @@ template <class P> NT2H_FD int reduce_d(double x, double* xr) {
 // converts the float to double and runs its double medium reduction (five
 // products, nine differences; fdlibm's double medium for the last float,
 // 0x1.921fb6p+19); above it NT2 calls fdlibm's float Payne-Hanek, for which
-// this build gives the correctly rounded xr instead.  Two FMAs with pi/2 in
+// these kernels give the correctly rounded xr instead.  Two FMAs with pi/2 in
 // two double pieces (the first exact: k*P1 and x are multiples of 2^-52 and
 // their difference is below 2) give, after rounding to float, NT2's xr and
 // quadrant on every float up to 2^18 pi and the correctly rounded xr with

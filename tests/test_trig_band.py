@@ -1,6 +1,6 @@
 """The EXACT float sin/cos band (2^6 pi, 2^28]: the GPU reduces these
 arguments with two double FMAs.  Up to 2^18 pi NT2 runs its double medium
-reduction (generic/rem_pio2_medium.hpp:68-83); above, this build gives the
+reduction (generic/rem_pio2_medium.hpp:68-83); above, the kernels give the
 correctly rounded reduced argument.  tests/cpp/trig_band_check compares the
 two-FMA reduction on every float of the band with the oracle's restatement
 of NT2's ladder and, above 2^18 pi, with a binary128 reduction; the kernels
