@@ -182,6 +182,10 @@ __global__ __launch_bounds__(kBlock) void bs_fwd_adj_f32_kernel(BSAdjArgs a, int
   }
 }
 
+// totals of an empty portfolio
+__global__ void zero_totals_kernel(double* t) {
+  if (threadIdx.x < 2) t[threadIdx.x] = 0.0;
+}
 constexpr unsigned kAdj64Blocks = 8192;  // workgroups of the FP64 adjoint kernel, at most
 template <bool TOTALS>
 __global__ __launch_bounds__(kBlock, 4) void bs_fwd_adj_f64_kernel(BSAdjArgsD a, int64_t n, Workspace ws,
@@ -251,10 +255,10 @@ int nt2h_bs_fwd_adj_f32(const float* S, const float* K, const float* T, const fl
   }
   if (n == 0) {
     if (totals) {
-      double z[2] = {0.0, 0.0};
       if (host) { totals[0] = 0.0; totals[1] = 0.0; return NT2H_OK; }
-      NT2H_HIP(hipMemcpyAsync(totals, z, sizeof z, hipMemcpyHostToDevice, as_hip(s)));
-      NT2H_HIP(hipStreamSynchronize(as_hip(s)));
+      // a kernel, not a copy from the host: asynchronous and capture-safe
+      hipLaunchKernelGGL(zero_totals_kernel, dim3(1), dim3(64), 0, as_hip(s), totals);
+      NT2H_HIP(hipGetLastError());
     }
     return NT2H_OK;
   }
@@ -288,10 +292,10 @@ int nt2h_bs_fwd_adj_f64(const double* S, const double* K, const double* T, const
   }
   if (n == 0) {
     if (totals) {
-      double z[2] = {0.0, 0.0};
       if (host) { totals[0] = 0.0; totals[1] = 0.0; return NT2H_OK; }
-      NT2H_HIP(hipMemcpyAsync(totals, z, sizeof z, hipMemcpyHostToDevice, as_hip(s)));
-      NT2H_HIP(hipStreamSynchronize(as_hip(s)));
+      // a kernel, not a copy from the host: asynchronous and capture-safe
+      hipLaunchKernelGGL(zero_totals_kernel, dim3(1), dim3(64), 0, as_hip(s), totals);
+      NT2H_HIP(hipGetLastError());
     }
     return NT2H_OK;
   }

@@ -182,3 +182,22 @@ def test_graph_lookback_scan_replays(dev):
             step()
         cs.synchronize()
         assert torch.equal(y, wy) and torch.equal(m, wm)
+
+
+def test_graph_empty_portfolio_totals(dev):
+    """The totals of an empty portfolio are written by a kernel, so the call
+    can be recorded: every replay zeroes them again."""
+    import torch
+
+    tot = torch.empty(2, dtype=torch.float64, device=dev)
+
+    def step():
+        nt2_amd.call("nt2h_bs_fwd_adj_f32", *([None] * 12), tot.data_ptr(), 0, _st())
+
+    g = _capture(step)
+    for _ in range(3):
+        tot.fill_(float("nan"))
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert tot.cpu().tolist() == [0.0, 0.0]

@@ -497,3 +497,21 @@ def test_bs_fwd_adj_on_coloured_tables(dev):
     b = ops.bs_fwd_adj(*col[:6], outs=col[6:])
     for x, y in zip(a, b):
         assert torch.equal(x, y)
+
+
+@pytest.mark.parametrize("fn", ["nt2h_bs_fwd_adj_f32", "nt2h_bs_fwd_adj_f64"])
+def test_bs_fwd_adj_empty_portfolio_totals(dev, fn):
+    """n = 0: no table is touched (null pointers allowed) and the totals are
+    zero, written on the stream for device totals and at once for host ones."""
+    import ctypes
+
+    import torch
+
+    tot = torch.full((2,), float("nan"), dtype=torch.float64, device=dev)
+    st = torch.cuda.current_stream().cuda_stream
+    nt2_amd.call(fn, *([None] * 12), tot.data_ptr(), 0, st)
+    torch.cuda.synchronize()
+    assert tot.cpu().tolist() == [0.0, 0.0]
+    host = (ctypes.c_double * 2)(float("nan"), float("nan"))
+    nt2_amd.call(fn, *([None] * 12), ctypes.addressof(host), 0, st)
+    assert list(host) == [0.0, 0.0]
