@@ -285,7 +285,8 @@ def test_long_run_scans(dev, ref, shape):
     assert np.all(np.abs(gp - wp) <= 2 * np.spacing(wp.astype(np.float32)))
 
 
-@pytest.mark.parametrize("shape", [(1, 1 << 19, 1), (1, (1 << 16) + 4, 3), (1, 20000, 1), (1, 16384, 2)])
+@pytest.mark.parametrize("shape", [(1, 1 << 19, 1), (1, (1 << 16) + 4, 3), (1, 20000, 1), (1, 16384, 2),
+                                   (1, 1 << 19, 4)])
 def test_lookback_scans(dev, ref, shape):
     """Runs of 2^14 to 2^19 floats are scanned in one launch, tiles finding
     their prefix from the published state of the tiles before them (a
