@@ -515,3 +515,28 @@ def test_bs_fwd_adj_empty_portfolio_totals(dev, fn):
     host = (ctypes.c_double * 2)(float("nan"), float("nan"))
     nt2_amd.call(fn, *([None] * 12), ctypes.addressof(host), 0, st)
     assert list(host) == [0.0, 0.0]
+
+
+@pytest.mark.parametrize("fn", ["sin", "cos"])
+def test_trig_exact_band_strided(dev, ref, fn):
+    """Every 257th float of the reduction band (2^6 pi, 2^28] and the floats at
+    its edges, both signs, through the aligned EXACT kernel (two-FMA band
+    reduction on every lane): bit-identical to the oracle's NT2 ladder up to
+    2^18 pi, within 2 ULP of it above (NT2 uses fdlibm's float Payne-Hanek
+    there, the kernels the correctly rounded reduced argument)."""
+    lo = np.float32(float.fromhex("0x1.921fb6p+7")).view(np.uint32)  # 2^6 pi (float)
+    mid = np.float32(float.fromhex("0x1.921fb6p+19")).view(np.uint32)  # 2^18 pi (float)
+    hi = np.float32(2.0 ** 28).view(np.uint32)
+    bits = np.concatenate([np.arange(lo - 2, hi + 3, 257, dtype=np.uint32),
+                           np.arange(lo - 2, lo + 3, dtype=np.uint32), np.arange(mid - 2, mid + 3, dtype=np.uint32),
+                           np.arange(hi - 2, hi + 3, dtype=np.uint32)])
+    x = bits.view(np.float32)
+    x = np.concatenate([x, -x])
+    x = np.concatenate([x, np.zeros((-x.size) % 4, np.float32)])  # whole 16-byte vectors
+    got = ops.map(fn, _gpu(x, dev), nt2_amd.EXACT).cpu().numpy()
+    want = ref.map_f32(fn, x)
+    near = np.abs(x) <= np.float32(float.fromhex("0x1.921fb6p+19"))
+    eq = _bits_equal(got[near], want[near])
+    assert eq.all(), (x[near][~eq][:8], got[near][~eq][:8], want[near][~eq][:8])
+    st = ref.cmp_f32(got[~near], want[~near])
+    assert st["max_std_ulp"] <= 2 and st["nan_mismatch"] == 0
