@@ -360,6 +360,38 @@ def report_folds(dev, stream):
     return lines
 
 
+def report_scans(dev, stream):
+    """Scans of medium runs, where launch count matters: runs of 2^14 to 2^19
+    floats (up to 512 tiles of 4096 floats in all) take one launch with
+    decoupled look-back, larger ones three passes (decimal GB/s of 8
+    B/element; times per call replayed from a HIP graph, and issued one by one
+    from Python for comparison)."""
+    import torch
+
+    from nt2_amd import SCAN_OPS, call
+    from nt2_amd.layout import streams
+
+    lines = []
+    n = 1 << 22
+    x, y = streams(2, n, torch.float32, dev)
+    x.normal_()
+    for lo, mid, hi in ((1, 1 << 14, 1), (1, 1 << 16, 1), (1, 1 << 18, 1), (1, 1 << 19, 1), (1, 1 << 16, 16),
+                        (1, 1 << 20, 1), (1, 1 << 16, 64)):
+        tiles = -(-mid // 4096)
+        path = "look-back" if tiles <= 128 and tiles * hi <= 512 else "three passes"
+        rec = {"report": "scans", "lo_mid_hi": [lo, mid, hi], "path": path}
+        for op in ("cumsum", "cummax", "cumprod"):
+            def scan():
+                call("nt2h_scan_f32", SCAN_OPS[op], x.data_ptr(), lo, mid, hi, y.data_ptr(),
+                     torch.cuda.current_stream().cuda_stream)
+            t = _graph_seconds(scan)  # GPU time per call: 100 calls replayed from one HIP graph
+            tp = _events(stream, scan, 200)  # the same calls issued one by one from Python
+            rec[op] = {"us": t * 1e6, "gbps": 8 * lo * mid * hi / t / 1e9, "python_calls_us": tp * 1e6}
+        lines.append(rec)
+    del x, y
+    return lines
+
+
 def report_host(dev, stream, n=1 << 24):
     """Host tables: staged pageable copies (nt2h_upload / nt2h_download) against
     hipMemcpy from pageable and from pinned memory, and the Black-Scholes
@@ -438,7 +470,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--report", default="", help="extra report lines after the headline: "
-                    "comma list of placement,sweep,bs_fwd,saxpy,colfold,folds,host (or all)")
+                    "comma list of placement,sweep,bs_fwd,saxpy,colfold,folds,scans,host (or all)")
     ap.add_argument("--sweep-fns", default="exp,log,sqrt,erfc,tanh,sin,cos",
                     help="functors of the precision sweep report")
     args = ap.parse_args()
@@ -620,11 +652,12 @@ def main():
     print(json.dumps(line), flush=True)
     want = set(args.report.split(",")) if args.report else set()
     if "all" in want:
-        want = {"saxpy", "bs_fwd", "sweep", "colfold", "placement", "folds", "host"}
+        want = {"saxpy", "bs_fwd", "sweep", "colfold", "placement", "folds", "scans", "host"}
     del g, outs, tabs, g64, outs64, tabs64
     torch.cuda.empty_cache()
     for name, fn in (("placement", report_placement), ("saxpy", report_saxpy), ("bs_fwd", report_bs_fwd),
-                     ("colfold", report_colfold), ("folds", report_folds), ("host", report_host),
+                     ("colfold", report_colfold), ("folds", report_folds), ("scans", report_scans),
+                     ("host", report_host),
                      ("sweep", report_sweep)):
         if name in want:
             recs = fn(dev, stream, fns=tuple(args.sweep_fns.split(","))) if name == "sweep" else fn(dev, stream)
