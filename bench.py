@@ -361,9 +361,9 @@ def report_folds(dev, stream):
 
 
 def report_scans(dev, stream):
-    """Scans of medium runs, where launch count matters: runs of 2^14 to 2^19
-    floats (up to 512 tiles of 4096 floats in all) take one launch with
-    decoupled look-back, larger ones three passes (decimal GB/s of 8
+    """Scans of medium runs, where launch count matters: runs of 2^14 to 2^20
+    floats (up to 512 tiles of 4096 or 8192 floats in all) take one launch
+    with decoupled look-back, larger ones three passes (decimal GB/s of 8
     B/element; times per call replayed from a HIP graph, and issued one by one
     from Python for comparison)."""
     import torch
@@ -372,12 +372,13 @@ def report_scans(dev, stream):
     from nt2_amd.layout import streams
 
     lines = []
-    n = 1 << 22
+    n = 1 << 23
     x, y = streams(2, n, torch.float32, dev)
     x.normal_()
-    for lo, mid, hi in ((1, 1 << 14, 1), (1, 1 << 16, 1), (1, 1 << 18, 1), (1, 1 << 19, 1), (1, 1 << 16, 16),
-                        (1, 1 << 20, 1), (1, 1 << 16, 64)):
-        tiles = -(-mid // 4096)
+    for lo, mid, hi in ((1, 1 << 14, 1), (1, 1 << 16, 1), (1, 1 << 18, 1), (1, 1 << 19, 1), (1, 1 << 20, 1),
+                        (1, 1 << 16, 16), (1, 1 << 16, 64), (1, 1 << 21, 1), (1, 1 << 16, 128)):
+        t4 = -(-mid // 4096)  # tiles of 4096 floats per run; 8192-float tiles past 64 per run or 256 in all
+        tiles = t4 if (t4 <= 64 and t4 * hi <= 256) else -(-mid // 8192)
         path = "look-back" if tiles <= 128 and tiles * hi <= 512 else "three passes"
         rec = {"report": "scans", "lo_mid_hi": [lo, mid, hi], "path": path}
         for op in ("cumsum", "cummax", "cumprod"):

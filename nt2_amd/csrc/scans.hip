@@ -38,7 +38,7 @@
 //    pass 2 the exclusive prefix of the segment totals per run (one
 //    workgroup per run; one lane per output on the column paths), pass 3
 //    rescans each segment from its prefix;
-//  * runs of 2^14 to 2^19 floats, 2^21 floats in all (16-byte vectors):
+//  * runs of 2^14 to 2^20 floats, 2^22 floats in all (16-byte vectors):
 //    one launch with decoupled look-back between tiles, the prefix of each
 //    tile folded in the order of the serial recurrence so that the results
 //    are reproducible.
@@ -680,7 +680,12 @@ __global__ __launch_bounds__(kPrefixBlock) void scan_prefix_kernel(int64_t segs,
 // anything.  Every wait is nevertheless bounded (spin_cap polls with a
 // sleep between them): past it the workgroup records an error in ctrl[3]
 // and finishes, so the kernel always ends.
-constexpr int kLbSub = 4;  // sub-tiles per look-back tile
+// Sub-tiles of kBlock x W elements per look-back tile (the kernel's kLbSub,
+// chosen per call): 4 (16 KiB) for short jobs, 8 (32 KiB: half the tickets
+// and half the chain) once the 16 KiB
+// tiles would exceed 64 per run or 256 in all (measured: 2^14 floats 7.6 us
+// with 4 against 9.8 with 8; 2^19 floats 18.3 against 15.9; 64 runs of 2^16
+// 26.7 us in three passes against 20.0 with 8)
 constexpr unsigned kLbSpinCap = 1u << 20;
 // Two limits, both measured against the three passes (three launches):
 // the tickets cost ~35 ns each, one after the other (16384 tiles: 887 us
@@ -704,7 +709,7 @@ __device__ __forceinline__ void lb_state(unsigned long long* f, unsigned long lo
 __device__ __forceinline__ unsigned long long lb_state_get(const unsigned long long* f) {
   return __hip_atomic_load(const_cast<unsigned long long*>(f), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-template <int OP, class T>
+template <int OP, class T, int kLbSub>
 __global__ __launch_bounds__(kBlock) void scan_lookback_kernel(SGeo<T> g, int64_t tpr, int64_t ntiles,
                                                                unsigned* ctrl, unsigned long long* state,
                                                                SSumm* agg, SSumm* incl, unsigned spin_cap) {
@@ -877,7 +882,9 @@ int scan_launch(const T* x, T* y, int64_t lo, int64_t mid, int64_t hi, nt2h_stre
     // up to kLbMaxTiles look-back tiles per run and kLbMaxTotal in all, with
     // 16-byte vectors: one pass with decoupled look-back (beyond: three
     // passes, see kLbMaxTiles)
-    const int64_t lbtile = (int64_t)kLbSub * kBlock * W;
+    const int64_t t4 = (mid + 4 * kBlock * W - 1) / (4 * kBlock * W);  // 16 KiB tiles per run
+    const int sub = (t4 > 64 || nout * t4 > 256) ? 8 : 4;
+    const int64_t lbtile = (int64_t)sub * kBlock * W;
     const int64_t tpr = (mid + lbtile - 1) / lbtile;
     if (vec && tpr <= kLbMaxTiles && nout * tpr <= kLbMaxTotal) {
       const int64_t ntiles = nout * tpr;
@@ -888,8 +895,12 @@ int scan_launch(const T* x, T* y, int64_t lo, int64_t mid, int64_t hi, nt2h_stre
       unsigned long long* lbst = reinterpret_cast<unsigned long long*>(reinterpret_cast<char*>(base) + 256);
       SSumm* agg = reinterpret_cast<SSumm*>(lbst + ntiles);
       SSumm* incl = agg + ntiles;
-      hipLaunchKernelGGL((scan_lookback_kernel<OP, T>), dim3((unsigned)ntiles), dim3(kBlock), 0, s, g, tpr, ntiles,
-                         ctrl, lbst, agg, incl, kLbSpinCap);
+      if (sub == 8)
+        hipLaunchKernelGGL((scan_lookback_kernel<OP, T, 8>), dim3((unsigned)ntiles), dim3(kBlock), 0, s, g, tpr,
+                           ntiles, ctrl, lbst, agg, incl, kLbSpinCap);
+      else
+        hipLaunchKernelGGL((scan_lookback_kernel<OP, T, 4>), dim3((unsigned)ntiles), dim3(kBlock), 0, s, g, tpr,
+                           ntiles, ctrl, lbst, agg, incl, kLbSpinCap);
       NT2H_HIP(hipGetLastError());
       return NT2H_OK;
     }
