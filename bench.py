@@ -380,7 +380,7 @@ def report_scans(dev, stream):
         t4 = -(-mid // 4096)  # tiles of 4096 floats per run; 8192-float tiles past 64 per run or 256 in all
         tiles = t4 if (t4 <= 64 and t4 * hi <= 256) else -(-mid // 8192)
         path = "look-back" if tiles <= 128 and tiles * hi <= 512 else "three passes"
-        rec = {"report": "scans", "lo_mid_hi": [lo, mid, hi], "path": path}
+        rec = {"report": "scans", "lo_mid_hi": [lo, mid, hi], "path": path}  # cumsum and cummax
         for op in ("cumsum", "cummax", "cumprod"):
             def scan():
                 call("nt2h_scan_f32", SCAN_OPS[op], x.data_ptr(), lo, mid, hi, y.data_ptr(),

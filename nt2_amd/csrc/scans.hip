@@ -886,7 +886,11 @@ int scan_launch(const T* x, T* y, int64_t lo, int64_t mid, int64_t hi, nt2h_stre
     const int sub = (t4 > 64 || nout * t4 > 256) ? 8 : 4;
     const int64_t lbtile = (int64_t)sub * kBlock * W;
     const int64_t tpr = (mid + lbtile - 1) / lbtile;
-    if (vec && tpr <= kLbMaxTiles && nout * tpr <= kLbMaxTotal) {
+    // cumprod's 32 KiB tiles hold 256 registers per lane (one wave per SIMD):
+    // half the limits (measured: three passes win from 128 such tiles on)
+    const bool heavy = OP == NT2H_SCAN_CUMPROD && sub == 8;
+    const int64_t max_run = heavy ? kLbMaxTiles / 2 : kLbMaxTiles, max_all = heavy ? kLbMaxTotal / 2 : kLbMaxTotal;
+    if (vec && tpr <= max_run && nout * tpr <= max_all) {
       const int64_t ntiles = nout * tpr;
       const size_t need = 256 + (size_t)ntiles * (sizeof(unsigned long long) + 2 * sizeof(SSumm));
       void* base = nullptr;
