@@ -314,6 +314,29 @@ def test_lookback_scans(dev, ref, shape):
         assert _bits_eq(_scan("cumprod", p, lo, mid, hi, dev), gp).all()
 
 
+@pytest.mark.parametrize("shape", [(3, (1 << 21) + 5, 1), (7, 1 << 19, 2)])
+def test_few_scalar_outputs_many_segments(dev, ref, shape):
+    """Outer folds and scans of tables with a few outputs per row that are not
+    a multiple of four (the scalar column kernels), split into tens of
+    thousands of segments: maximum/minimum/cummax/cummin bit-identical to
+    NT2's recurrence, sum and cumsum within the bounds of the other paths."""
+    lo, mid, hi = shape
+    x = _specials_data(lo * mid * hi, 51)
+    for op in ("maximum", "minimum"):
+        assert _bits_eq(_fold(op, x, lo, mid, hi, dev), ref.fold_lmh(op, x, lo, mid, hi)).all(), op
+    for op in ("cummax", "cummin"):
+        assert _bits_eq(_scan(op, x, lo, mid, hi, dev), ref.scan_lmh(op, x, lo, mid, hi)).all(), op
+    z = np.random.default_rng(52).standard_normal(lo * mid * hi).astype(np.float32)
+    z64 = z.astype(np.float64).reshape(hi, mid, lo)
+    gs = _fold("sum", z, lo, mid, hi, dev).astype(np.float64)
+    ws, ms = z64.sum(axis=1).reshape(-1), np.abs(z64).sum(axis=1).reshape(-1)
+    assert np.all(np.abs(gs - ws) <= np.spacing(np.abs(ws).astype(np.float32)) + 1e-12 * ms)
+    gz = _scan("cumsum", z, lo, mid, hi, dev)
+    wz = np.cumsum(z64, axis=1).reshape(-1)
+    mag = np.cumsum(np.abs(z64), axis=1).reshape(-1)
+    assert np.all(np.abs(gz - wz) <= np.spacing(np.abs(wz).astype(np.float32)) + 1e-12 * mag)
+
+
 def test_scan_unit_vectors(dev, ref):
     """modules/core/swar/unit/table/cumsum.cpp: reshape(_(1, 30), 3, 5, 2) along 1 and 2."""
     a = np.arange(1, 31, dtype=np.float32)
