@@ -297,23 +297,29 @@ template <class P> NT2H_FD float erf_f(float a0) {
 }
 // NT2 double erfc / erf (scalar/erfc.hpp:39-74, scalar/erf.hpp:39-71,
 // erf_kernel.hpp:91-183): four intervals split at 0.65, 2.2 and 6.
-template <class P> NT2H_FD double erf1_d(double xx) {
-  return P::div(horner<P>(xx, 0x1.10512d5b20332p-14, 0x1.3b7664358865ap-10, 0x1.4a59a4f02579cp-5,
-                          0x1.16500f106c0a5p-3, 0x1.20dd750429b61p+0),
-                horner<P>(xx, 0x1.7ea4332348252p-12, 0x1.166f75999dbd1p-7, 0x1.64536ca92ea2fp-4,
-                          0x1.d0a84eb1ca867p-2, 0x1.0p+0));
+template <class P> NT2H_FD double erf1_num_d(double xx) {
+  return horner<P>(xx, 0x1.10512d5b20332p-14, 0x1.3b7664358865ap-10, 0x1.4a59a4f02579cp-5,
+                   0x1.16500f106c0a5p-3, 0x1.20dd750429b61p+0);
 }
-template <class P> NT2H_FD double erfc2_d(double x) {
-  return P::div(horner<P>(x, 0.0, 0x1.cf4cfe0aacbb4p-8, 0x1.2488a6b5cb5e5p-4, 0x1.53dd7a67c7e9fp-2,
-                          0x1.c1986509e687bp-1, 0x1.54dfe9b258a60p+0, 0x1.ffffffbbb552bp-1),
-                horner<P>(x, 0x1.9a996639b0d00p-7, 0x1.033c113a7deeep-3, 0x1.307622fcff772p-1,
-                          0x1.9e677c2777c3cp+0, 0x1.53b1052dca8bdp+1, 0x1.3adeae79b9708p+1, 0x1.0p+0));
+template <class P> NT2H_FD double erf1_den_d(double xx) {
+  return horner<P>(xx, 0x1.7ea4332348252p-12, 0x1.166f75999dbd1p-7, 0x1.64536ca92ea2fp-4,
+                   0x1.d0a84eb1ca867p-2, 0x1.0p+0);
 }
-template <class P> NT2H_FD double erfc3_d(double x) {
-  return P::div(horner<P>(x, 0.0, 0x1.71d0907ea7a92p-6, 0x1.42210f88b9d43p-3, 0x1.29be1cff90d94p-1,
-                          0x1.44744306832aep+0, 0x1.9fa202deb88e5p+0, 0x1.fff5a9e697ae2p-1),
-                horner<P>(x, 0x1.47bd61bbb3843p-5, 0x1.1d7ab774bb837p-2, 0x1.0cfd4cb6cde9fp+0,
-                          0x1.315ffdfd5ce91p+1, 0x1.afd487397568fp+1, 0x1.602f24bf3fdb6p+1, 0x1.0p+0));
+template <class P> NT2H_FD double erfc2_num_d(double x) {
+  return horner<P>(x, 0.0, 0x1.cf4cfe0aacbb4p-8, 0x1.2488a6b5cb5e5p-4, 0x1.53dd7a67c7e9fp-2,
+                   0x1.c1986509e687bp-1, 0x1.54dfe9b258a60p+0, 0x1.ffffffbbb552bp-1);
+}
+template <class P> NT2H_FD double erfc2_den_d(double x) {
+  return horner<P>(x, 0x1.9a996639b0d00p-7, 0x1.033c113a7deeep-3, 0x1.307622fcff772p-1,
+                   0x1.9e677c2777c3cp+0, 0x1.53b1052dca8bdp+1, 0x1.3adeae79b9708p+1, 0x1.0p+0);
+}
+template <class P> NT2H_FD double erfc3_num_d(double x) {
+  return horner<P>(x, 0.0, 0x1.71d0907ea7a92p-6, 0x1.42210f88b9d43p-3, 0x1.29be1cff90d94p-1,
+                   0x1.44744306832aep+0, 0x1.9fa202deb88e5p+0, 0x1.fff5a9e697ae2p-1);
+}
+template <class P> NT2H_FD double erfc3_den_d(double x) {
+  return horner<P>(x, 0x1.47bd61bbb3843p-5, 0x1.1d7ab774bb837p-2, 0x1.0cfd4cb6cde9fp+0,
+                   0x1.315ffdfd5ce91p+1, 0x1.afd487397568fp+1, 0x1.602f24bf3fdb6p+1, 0x1.0p+0);
 }
 template <class P> NT2H_FD double erfc4_d(double rx) {
   return horner<P>(rx, -0x1.05fce04ec4ec5p+6, 0x1.5803d26c4ec4fp+6, -0x1.57e03041e9d8bp+5,
@@ -321,39 +327,71 @@ template <class P> NT2H_FD double erfc4_d(double rx) {
                    0x1.4a451701654f1p-12, 0x1.b1494cac06d39p-2, 0x1.07e71e046a820p-22, -0x1.20dd753ae5dfdp-2,
                    0x1.60000e984b501p-36, 0x1.20dd750429a16p-1, -0x1.e4ad1ec7d0000p-56);
 }
-// erfc(x) for x = |a0| >= 0 (double); e2 receives exp(-x^2) when computed.
+// erfc(x) for x = |a0| >= 0 (double); e2 receives exp(-x^2) when computed
+// (1 for x <= 1e-10).  In a wave of mixed arguments each interval's branch runs
+// under its own lane mask, yet every lane needs exactly one quotient: the
+// intervals form their numerator and denominator (the far interval divides 1 by
+// x ahead of its polynomial in 1/x) and one division serves them all, and
+// exp(-x^2) is computed once.  Each lane performs NT2's operations on NT2's
+// operands, so the values are those of the per-interval formulas.
 template <class P> NT2H_FD double erfc_pos_d(double x, double* e2 = nullptr) {
-  double xx = x * x;
-  double z;
-  if (x <= 0.0000000001) {
-    z = 1.0 - P::div(x * 2.0, 0x1.c5bf891b4ef6ap+0);  // sqrt(pi)
-    if (e2) *e2 = 1.0;
-  } else if (x < 0.65) {
-    z = 1.0 - x * erf1_d<P>(xx);
-    if (e2) *e2 = exp_d<P>(-xx);
+  const double xx = x * x;
+  const bool tiny = x <= 0.0000000001, small = x < 0.65, mid = x < 2.2, near = x < 6.0;
+  double num, den;
+  if (tiny) {
+    num = x * 2.0;
+    den = 0x1.c5bf891b4ef6ap+0;  // sqrt(pi)
+  } else if (small) {
+    num = erf1_num_d<P>(xx);
+    den = erf1_den_d<P>(xx);
+  } else if (mid) {
+    num = erfc2_num_d<P>(x);
+    den = erfc2_den_d<P>(x);
+  } else if (near) {
+    num = erfc3_num_d<P>(x);
+    den = erfc3_den_d<P>(x);
   } else {
-    double ex = exp_d<P>(-xx);
-    if (e2) *e2 = ex;
-    if (x < 2.2) z = ex * erfc2_d<P>(x);
-    else if (x < 6.0) z = ex * erfc3_d<P>(x);
-    else z = ex * erfc4_d<P>(P::rcp(x));
+    num = 1.0;  // P::rcp(x)
+    den = x;
   }
-  return z;
+  double q = P::div(num, den);
+  if (!near) q = erfc4_d<P>(q);
+  double ex = 1.0;
+  if (!small || (e2 != nullptr && !tiny)) ex = exp_d<P>(-xx);
+  if (e2) *e2 = ex;
+  return small ? 1.0 - (tiny ? q : x * q) : ex * q;
 }
 template <class P> NT2H_FD double erfc_d(double a0) {
   double z = erfc_pos_d<P>(__builtin_fabs(a0));
   z = (a0 < 0.0) ? 2.0 - z : z;
   return (a0 != a0) ? a0 : z;
 }
+// Same intervals as NT2's scalar erf, with one division and one exp(-x^2)
+// serving every lane of a wave (as in erfc_pos_d).
 template <class P> NT2H_FD double erf_d(double a0) {
-  double x = __builtin_fabs(a0);
-  double xx = x * x;
-  double r;
-  if (x <= 0.0000000001) r = P::div(a0 * 2.0, 0x1.c5bf891b4ef6ap+0);
-  else if (x <= 0.65) r = a0 * erf1_d<P>(xx);
-  else if (x <= 2.2) { double z = 1.0 - exp_d<P>(-xx) * erfc2_d<P>(x); r = a0 < 0.0 ? -z : z; }
-  else if (x <= 6.0) { double z = 1.0 - exp_d<P>(-xx) * erfc3_d<P>(x); r = a0 < 0.0 ? -z : z; }
-  else r = 1.0;
+  const double x = __builtin_fabs(a0);
+  const double xx = x * x;
+  const bool tiny = x <= 0.0000000001, small = x <= 0.65, mid = x <= 2.2, near = x <= 6.0;
+  double num = 1.0, den = 1.0;
+  if (tiny) {
+    num = a0 * 2.0;
+    den = 0x1.c5bf891b4ef6ap+0;  // sqrt(pi)
+  } else if (small) {
+    num = erf1_num_d<P>(xx);
+    den = erf1_den_d<P>(xx);
+  } else if (mid) {
+    num = erfc2_num_d<P>(x);
+    den = erfc2_den_d<P>(x);
+  } else if (near) {
+    num = erfc3_num_d<P>(x);
+    den = erfc3_den_d<P>(x);
+  }
+  const double q = P::div(num, den);
+  double ex = 1.0;
+  if (!small && near) ex = exp_d<P>(-xx);
+  double z = 1.0 - ex * q;
+  z = a0 < 0.0 ? -z : z;
+  const double r = tiny ? q : small ? a0 * q : near ? z : 1.0;
   return (a0 != a0) ? a0 : r;
 }
 
