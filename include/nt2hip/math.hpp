@@ -441,19 +441,25 @@ template <class P> NT2H_FD float tanh_f(float a0) {
   big = __uint_as_float(__float_as_uint(big) ^ (__float_as_uint(a0) & 0x80000000u));
   return (x < 0.625f) ? small : big;
 }
+// Both branches end in a division (x^2 P/Q below 0.625, 1/(1+exp(2x)) above);
+// each lane forms its operands under its branch and one division serves the
+// wave, with NT2's operations on NT2's operands in every lane.
 template <class P> NT2H_FD double tanh_d(double a0) {
-  double x = __builtin_fabs(a0);
-  double r;
-  if (x < 0.625) {
-    double x2 = x * x;
-    double p = horner<P>(x2, -0x1.edc5baafd6f4bp-1, -0x1.8d26a0e26682dp+6, -0x1.93ac030580563p+10);
-    double q = horner<P>(x2, 0x1.0p+0, 0x1.c33f28a581b86p+6, 0x1.176fa0e5535fap+11, 0x1.2ec102442040cp+12);
-    r = P::madd(P::div(x2 * p, q), a0, a0);
+  const double x = __builtin_fabs(a0);
+  const bool small = x < 0.625;
+  double num, den;
+  if (small) {
+    const double x2 = x * x;
+    num = x2 * horner<P>(x2, -0x1.edc5baafd6f4bp-1, -0x1.8d26a0e26682dp+6, -0x1.93ac030580563p+10);
+    den = horner<P>(x2, 0x1.0p+0, 0x1.c33f28a581b86p+6, 0x1.176fa0e5535fap+11, 0x1.2ec102442040cp+12);
   } else {
-    double b = P::madd(-2.0, P::rcp(1.0 + exp_d<P>(x + x)), 1.0);
-    r = __longlong_as_double(__double_as_longlong(b) ^ (__double_as_longlong(a0) & (long long)0x8000000000000000ull));
+    num = 1.0;  // P::rcp(1 + exp(2x))
+    den = 1.0 + exp_d<P>(x + x);
   }
-  return r;
+  const double q = P::div(num, den);
+  if (small) return P::madd(q, a0, a0);
+  const double b = P::madd(-2.0, q, 1.0);
+  return __longlong_as_double(__double_as_longlong(b) ^ (__double_as_longlong(a0) & (long long)0x8000000000000000ull));
 }
 
 // ========================================================== trigonometry
