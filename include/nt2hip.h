@@ -225,6 +225,26 @@ int nt2h_bs_fwd_adj_f32(const float* S, const float* K, const float* T, const fl
 int nt2h_bs_fwd_adj_f32_host(const float* S, const float* K, const float* T, const float* r, const float* sig,
                              const float* w, float* price, float* dS, float* dK, float* dT, float* dr, float* dsig,
                              double* totals, int64_t n, int64_t block);
+/* No counterpart in NT2: nt2h_bs_fwd_adj_f32 for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The same kernel, with its reduction
+ * workspace in a static device array of the library (about 325 KiB per GPU,
+ * held until the process ends) instead of a per-stream allocation: the call
+ * makes no hipMalloc/hipFree.  Outputs and totals are bit-identical to
+ * nt2h_bs_fwd_adj_f32's for 16-byte aligned tables.
+ *  - n <= 2^24 options (16384 workgroups' partials); larger n returns
+ *    NT2H_ERR_TOO_LARGE (split the portfolio and add the totals).
+ *  - totals is NULL or device memory (a host pointer: NT2H_ERR_INVALID_ARG).
+ *  - Every call on one GPU shares that GPU's array: calls that overlap in
+ *    time on one GPU (other streams, other host threads) race and give wrong
+ *    totals.  Calls in stream order are safe; each launch leaves the array's
+ *    ticket counters at zero for the next.
+ *  - The array is resolved and cleared by nt2h_init (or by the first call on
+ *    a device nt2h_init did not see; that first call is not capturable and
+ *    returns NT2H_ERR_CAPTURE while its stream is capturing). */
+int nt2h_bs_fwd_adj_f32_noalloc(const float* S, const float* K, const float* T, const float* r,
+                                const float* sig, const float* w, float* price, float* dS, float* dK, float* dT,
+                                float* dr, float* dsig, double* totals, int64_t n, nt2h_stream_t s);
 int nt2h_bs_fwd_adj_f64(const double* S, const double* K, const double* T, const double* r,
                         const double* sig, const double* w, double* price, double* dS, double* dK, double* dT,
                         double* dr, double* dsig, double* totals, int64_t n, nt2h_stream_t s);

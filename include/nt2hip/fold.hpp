@@ -125,7 +125,9 @@ __device__ __forceinline__ double fetch(const double* p) {
 // publishes the block's partials and takes a ticket on its group of kGroup
 // blocks; the block holding the group's last ticket sums the group's
 // partials in block order (one lane per block, fixed shuffle tree), publishes
-// the group partial and takes a ticket on the top counter; the last group
+// the group partial and takes a ticket on the top counter (slot ng, the first
+// one no group of this launch uses, so a workspace needs only ng + 1
+// counters); the last group
 // sums the group partials in group order.  Every ticket counter is reset by
 // the block that consumed it.  Returns true in lane 0 of the final block,
 // with the totals in res[0..NV).
@@ -171,7 +173,7 @@ __device__ __forceinline__ bool grid_combine(const double (&bsum)[NV], const Wor
     for (int v = 0; v < NV; ++v) publish(&ws.gpartials[v * ng + g], acc[v]);
     __hip_atomic_store(&ws.counters[g * kCounterStride], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __builtin_amdgcn_s_waitcnt(0);
-    t2 = __hip_atomic_fetch_add(&ws.counters[kMaxGroups * kCounterStride], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    t2 = __hip_atomic_fetch_add(&ws.counters[ng * kCounterStride], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   t2 = __shfl(t2, 0, 64);
   if (t2 != ng - 1) return false;
@@ -201,7 +203,7 @@ __device__ __forceinline__ bool grid_combine(const double (&bsum)[NV], const Wor
   if (lane == 0) {
 #pragma unroll
     for (int v = 0; v < NV; ++v) res[v] = acc[v];
-    __hip_atomic_store(&ws.counters[kMaxGroups * kCounterStride], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&ws.counters[ng * kCounterStride], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   return lane == 0;
 }

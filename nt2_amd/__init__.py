@@ -59,6 +59,7 @@ SIGNATURES = {
     "nt2h_dot_f64": (_I, [_P, _P, _I64, _P, _P]),
     "nt2h_bs_fwd_adj_f32": (_I, [_P] * 13 + [_I64, _P]),
     "nt2h_bs_fwd_adj_f64": (_I, [_P] * 13 + [_I64, _P]),
+    "nt2h_bs_fwd_adj_f32_noalloc": (_I, [_P] * 13 + [_I64, _P]),
     "nt2h_colfold_gc_f32": (_I, [_P, _I64, _I64, _I64] + [_P] * 8 + [_P]),
     "nt2h_fold_f32": (_I, [_I, _P, _I64, _I64, _I64, _P, _P]),
     "nt2h_fold_f64": (_I, [_I, _P, _I64, _I64, _I64, _P, _P]),

@@ -23,6 +23,8 @@
 // and combined across workgroups in FP64 by the last workgroup.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "nt2h_common.hpp"
 #include "../../include/nt2hip/math.hpp"
 
@@ -234,6 +236,54 @@ __global__ __launch_bounds__(kBlock, 4) void bs_fwd_adj_f64_kernel(BSAdjArgsD a,
   }
 }
 
+// ------------------------------------------------ allocation-free variant
+// Scratch of nt2h_bs_fwd_adj_f32_noalloc: the reduction workspace of at most
+// kNaBlocks workgroups (the grid of 2^24 options, one 16-byte vector per lane
+// per stream) in one static device array, so that the call allocates nothing.
+//   [0, 256 KiB)         block partials, 2 x kNaBlocks doubles
+//   [256 KiB, +4 KiB)    group partials, 2 x kNaGroups doubles
+//   then                 kNaGroups + 1 ticket counters, kCounterStride apart
+//   then                 2 doubles (result slot, unused: totals go to device memory)
+// Each device holds its own copy of the array until the process ends.  Host
+// code never uses the array's name as an address: the per-device address
+// comes from hipGetSymbolAddress (noalloc_setup).
+constexpr int kNaBlocks = 16384;
+constexpr int kNaGroups = kNaBlocks / kGroup;
+constexpr int64_t kNaMaxOptions = (int64_t)kNaBlocks * kBlock * 4;  // 2^24
+constexpr size_t kNaPartialsOff = 0;
+constexpr size_t kNaGroupOff = kNaPartialsOff + 2 * sizeof(double) * kNaBlocks;
+constexpr size_t kNaCountersOff = kNaGroupOff + 2 * sizeof(double) * kNaGroups;
+constexpr size_t kNaCountersBytes = sizeof(unsigned) * (kNaGroups + 1) * kCounterStride;
+constexpr size_t kNaResultOff = (kNaCountersOff + kNaCountersBytes + 255) / 256 * 256;
+constexpr size_t kNaBytes = (kNaResultOff + 2 * sizeof(double) + 255) / 256 * 256;
+static_assert(kNaCountersOff % 256 == 0, "counters start on a 256-byte boundary");
+
+static __device__ __align__(256) unsigned char scratch[kNaBytes];
+
+namespace {
+constexpr int kNaMaxDevices = 64;
+std::mutex g_na_mutex;
+void* g_na_ptr[kNaMaxDevices] = {};
+}  // namespace
+
+// Once per device, with that device current: the device address of
+// `scratch` on it, and its ticket counters cleared (every later launch
+// returns the counters it used to zero).  Called by nt2h_init and, for a
+// device nt2h_init never saw, by the entry point's first call there.
+int noalloc_setup(int dev, void** out) {
+  if (dev < 0 || dev >= kNaMaxDevices) return NT2H_ERR_NO_DEVICE;
+  std::lock_guard<std::mutex> lock(g_na_mutex);
+  if (!g_na_ptr[dev]) {
+    void* p = nullptr;
+    NT2H_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    NT2H_HIP(hipMemset(static_cast<unsigned char*>(p) + kNaCountersOff, 0, kNaCountersBytes));
+    NT2H_HIP(hipDeviceSynchronize());
+    g_na_ptr[dev] = p;
+  }
+  if (out) *out = g_na_ptr[dev];
+  return NT2H_OK;
+}
+
 }  // namespace nt2h
 
 using namespace nt2h;
@@ -308,6 +358,61 @@ int nt2h_bs_fwd_adj_f64(const double* S, const double* K, const double* T, const
   else hipLaunchKernelGGL((bs_fwd_adj_f64_kernel<false>), dim3(g), dim3(kBlock), 0, as_hip(s), a, n, ws, dst);
   NT2H_HIP(hipGetLastError());
   return totals ? deliver(s, totals, ws.result, 2, host) : NT2H_OK;
+}
+
+int nt2h_bs_fwd_adj_f32_noalloc(const float* S, const float* K, const float* T, const float* r,
+                                const float* sig, const float* w, float* price, float* dS, float* dK, float* dT,
+                                float* dr, float* dsig, double* totals, int64_t n, nt2h_stream_t s) {
+  NT2H_TRY(check_n(n));
+  NT2H_TRY(check_ptrs(n, S, K, T, r, sig, w, price, dS, dK, dT, dr, dsig));
+  if (n > kNaMaxOptions) return NT2H_ERR_TOO_LARGE;
+  if (totals) {
+    bool host = false;
+    NT2H_TRY(is_host_pointer(totals, &host, s));
+    if (host) return NT2H_ERR_INVALID_ARG;  // the result is written by the kernel: device memory only
+  }
+  int dev = 0;
+  NT2H_HIP(hipGetDevice(&dev));
+  void* base = nullptr;
+  {
+    // set-up is not capturable (a memset and a device synchronisation)
+    std::lock_guard<std::mutex> lock(g_na_mutex);
+    if (dev >= 0 && dev < kNaMaxDevices) base = g_na_ptr[dev];
+  }
+  if (!base) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(as_hip(s), &cs) == hipSuccess && cs != hipStreamCaptureStatusNone)
+      return NT2H_ERR_CAPTURE;
+    NT2H_TRY(noalloc_setup(dev, &base));
+  }
+  if (n == 0) {
+    if (totals) {
+      hipLaunchKernelGGL(zero_totals_kernel, dim3(1), dim3(64), 0, as_hip(s), totals);
+      NT2H_HIP(hipGetLastError());
+    }
+    return NT2H_OK;
+  }
+  unsigned char* b = static_cast<unsigned char*>(base);
+  Workspace ws{};
+  ws.partials = reinterpret_cast<double*>(b + kNaPartialsOff);
+  ws.gpartials = reinterpret_cast<double*>(b + kNaGroupOff);
+  ws.counters = reinterpret_cast<unsigned*>(b + kNaCountersOff);
+  ws.result = reinterpret_cast<double*>(b + kNaResultOff);
+  ws.bpartials = nullptr;
+  BSAdjArgs a{{S, K, T, r, sig, w}, {price, dS, dK, dT, dr, dsig}};
+  bool vec = aligned16(S, K, T, r, sig, w, price, dS, dK, dT, dr, dsig);
+  unsigned g = fold_blocks(vec ? (n + 3) / 4 : n);
+  if (g > (unsigned)kNaBlocks) g = kNaBlocks;  // the scalar body grid-strides; the vector grid fits already
+  hipStream_t st = as_hip(s);
+  if (vec) {
+    if (totals) hipLaunchKernelGGL((bs_fwd_adj_f32_kernel<true, true>), dim3(g), dim3(kBlock), 0, st, a, n, ws, totals);
+    else hipLaunchKernelGGL((bs_fwd_adj_f32_kernel<true, false>), dim3(g), dim3(kBlock), 0, st, a, n, ws, totals);
+  } else {
+    if (totals) hipLaunchKernelGGL((bs_fwd_adj_f32_kernel<false, true>), dim3(g), dim3(kBlock), 0, st, a, n, ws, totals);
+    else hipLaunchKernelGGL((bs_fwd_adj_f32_kernel<false, false>), dim3(g), dim3(kBlock), 0, st, a, n, ws, totals);
+  }
+  NT2H_HIP(hipGetLastError());
+  return NT2H_OK;
 }
 
 }  // extern "C"

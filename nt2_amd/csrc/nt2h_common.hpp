@@ -62,5 +62,8 @@ int get_lookback(nt2h_stream_t s, size_t bytes, void** out);
 // host memory); for host memory the stream is synchronised.
 int deliver(nt2h_stream_t s, double* out, const double* dev, int nv, bool out_is_host);
 int is_host_pointer(const void* p, bool* host, nt2h_stream_t s);
+// Device address of the allocation-free entry point's static scratch on
+// device dev (which must be current), resolved and cleared once per device.
+int noalloc_setup(int dev, void** out);
 
 }  // namespace nt2h

@@ -180,6 +180,9 @@ int nt2h_init(int device) {
   if (device < 0 || device >= count) return NT2H_ERR_NO_DEVICE;
   NT2H_HIP(hipSetDevice(device));
   NT2H_HIP(hipFree(nullptr));  // create the context now rather than on first launch
+  // the static scratch of the allocation-free entry point, resolved here so
+  // that its callers never meet the set-up (a memset and a synchronisation)
+  NT2H_TRY(noalloc_setup(device, nullptr));
   return NT2H_OK;
 }
 
