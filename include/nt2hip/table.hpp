@@ -8,6 +8,7 @@
 //   float s = globalsum(X * Y);                  // nt2h_dot_f32
 //   table<float> z = sum(exp(-sqr(A)) * cos(sx(k) * A), 1);   // inner_fold over rows
 //   tie(call, put) = blackscholes(S, K, T, r, sig);           // 5 -> 2 streams, one launch
+//   tie(da, d1, d2, R) = tie(sqrt(T), ..., fnms(v, da, d1), ...); // N outputs, one kernel
 //
 // It stands where NT2's container DSL does: table<T,S> (core/container/
 // table/table.hpp:34-171), table::operator= -> evaluate -> run_assign
@@ -15,7 +16,9 @@
 // run.hpp:21-35), the transform loop nest (core/functions/transform.hpp:
 // 55-69), terminal load/store (dsl/functions/container/terminal.hpp:28-129),
 // the per-node recursion run(node, i) (dsl/functions/generic/run.hpp:
-// 128-222), tie (core/functions/expr/tie.hpp:23-160), the reductions
+// 128-222), the multiply-adds fma/fms/fnms/fnma (boost/simd/arithmetic/
+// functions/generic/{fma,fms,fnms,fnma}.hpp), the N-ary tie (core/functions/
+// expr/tie.hpp:23-160), the reductions
 // (dsl/functions/reduction/run.hpp:30-104: inner_fold along dimension 1,
 // outer_fold along dimension 2) and the sx broadcast (core/functions/sx.hpp).
 //
@@ -105,6 +108,7 @@ template <class T> struct term {  // a table read at the element index
     return q;
   }
   bool aligned() const { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+  bool bcast_reads(const void*) const { return false; }
 };
 template <class T> struct row_bcast {  // sx: a 1 x cols row repeated down every column
   const T* p;
@@ -123,12 +127,14 @@ template <class T> struct row_bcast {  // sx: a 1 x cols row repeated down every
     return q;
   }
   bool aligned() const { return true; }
+  bool bcast_reads(const void* q) const { return q == static_cast<const void*>(p); }
 };
 template <class T> struct scalar {
   T v;
   template <class P> NT2H_FD T at(int64_t, int64_t) const { return v; }
   template <class P> NT2H_FD quad<T> at4(int64_t, int64_t) const { return quad<T>{{v, v, v, v}}; }
   bool aligned() const { return true; }
+  bool bcast_reads(const void*) const { return false; }
 };
 template <class F, class A> struct unary {
   A a;
@@ -141,6 +147,7 @@ template <class F, class A> struct unary {
     return q;
   }
   bool aligned() const { return a.aligned(); }
+  bool bcast_reads(const void* p) const { return a.bcast_reads(p); }
 };
 template <class F, class A, class B> struct binary {
   A a;
@@ -157,6 +164,26 @@ template <class F, class A, class B> struct binary {
     return q;
   }
   bool aligned() const { return a.aligned() && b.aligned(); }
+  bool bcast_reads(const void* p) const { return a.bcast_reads(p) || b.bcast_reads(p); }
+};
+template <class F, class A, class B, class C> struct ternary {
+  A a;
+  B b;
+  C c;
+  template <class P> NT2H_FD auto at(int64_t i, int64_t r) const {
+    return F::template apply<P>(a.template at<P>(i, r), b.template at<P>(i, r), c.template at<P>(i, r));
+  }
+  template <class P> NT2H_FD auto at4(int64_t i, int64_t r) const {
+    const auto x = a.template at4<P>(i, r);
+    const auto y = b.template at4<P>(i, r);
+    const auto z = c.template at4<P>(i, r);
+    quad<decltype(F::template apply<P>(x.v[0], y.v[0], z.v[0]))> q;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) q.v[k] = F::template apply<P>(x.v[k], y.v[k], z.v[k]);
+    return q;
+  }
+  bool aligned() const { return a.aligned() && b.aligned() && c.aligned(); }
+  bool bcast_reads(const void* p) const { return a.bcast_reads(p) || b.bcast_reads(p) || c.bcast_reads(p); }
 };
 }  // namespace node
 
@@ -218,7 +245,21 @@ struct multiplies_ {
 struct divides_ {
   template <class P, class T> NT2H_FD static T apply(T a, T b) { return P::div(a, b); }
 };
-struct fma_ {};  // built through fma(a, b, c) below
+// The multiply-adds of Boost.SIMD (arithmetic/functions/generic/{fma,fms,
+// fnma,fnms}.hpp:32): NT2's scalar path forms the product and rounds twice;
+// the fused policy issues one v_fma.
+struct fma_ {  // a*b + c
+  template <class P, class T> NT2H_FD static T apply(T a, T b, T c) { return P::madd(a, b, c); }
+};
+struct fms_ {  // a*b - c
+  template <class P, class T> NT2H_FD static T apply(T a, T b, T c) { return P::madd(a, b, -c); }
+};
+struct fnms_ {  // c - a*b
+  template <class P, class T> NT2H_FD static T apply(T a, T b, T c) { return P::nms(a, b, c); }
+};
+struct fnma_ {  // -c - a*b
+  template <class P, class T> NT2H_FD static T apply(T a, T b, T c) { return P::nms(a, b, -c); }
+};
 }  // namespace fn
 
 // ==================================================== expression wrapper
@@ -313,6 +354,42 @@ NT2HIP_BOP(*, multiplies_)
 NT2HIP_BOP(/, divides_)
 #undef NT2HIP_BOP
 
+// ------------------------------------------------------- ternary builders
+// fma(a, b, c), fms, fnms, fnma with any mix of tables, expressions and
+// scalars (NT2's demo writes fma(sqr(v), T(0.5), r) and fnms(v, da, d1)).
+template <class X> struct is_operand : std::integral_constant<bool, is_table<X>::value || is_expr<X>::value> {};
+template <class... Xs> struct first_value;
+template <class X, class... R> struct first_value<X, R...> {
+  using type = typename std::conditional<is_operand<X>::value, value_of<X>, first_value<R...>>::type::type;
+};
+template <class T, class X> inline auto operand(const X& x) {
+  if constexpr (is_operand<X>::value) {
+    static_assert(std::is_same<typename value_of<X>::type, T>::value, "operands of one value type");
+    return as_expr(x);
+  } else {
+    return scal<T>(x);
+  }
+}
+template <class F, class A, class B, class C> inline auto make_ter(const A& a, const B& b, const C& c) {
+  using T = typename first_value<A, B, C>::type;
+  const auto ea = operand<T>(a);
+  const auto eb = operand<T>(b);
+  const auto ec = operand<T>(c);
+  using N = node::ternary<F, decltype(ea.n), decltype(eb.n), decltype(ec.n)>;
+  return expr<T, N>{N{ea.n, eb.n, ec.n}, merge<T>(merge<T>(ea.ext, eb.ext), ec.ext)};
+}
+#define NT2HIP_TFUN(NAME, F)                                                                              \
+  template <class A, class B, class C,                                                                    \
+            class = std::enable_if_t<is_operand<A>::value || is_operand<B>::value || is_operand<C>::value>> \
+  inline auto NAME(const A& a, const B& b, const C& c) {                                                  \
+    return make_ter<fn::F>(a, b, c);                                                                      \
+  }
+NT2HIP_TFUN(fma, fma_)
+NT2HIP_TFUN(fms, fms_)
+NT2HIP_TFUN(fnms, fnms_)
+NT2HIP_TFUN(fnma, fnma_)
+#undef NT2HIP_TFUN
+
 // sx(row): a 1 x cols table repeated down the rows of whatever (rows x cols)
 // operand it meets (core/functions/sx.hpp); on its own it is 1 x cols.
 template <class T> inline expr<T, node::row_bcast<T>> sx(const table<T>& row) {
@@ -350,43 +427,73 @@ __global__ __launch_bounds__(256) void k_transform4(N n, T* out, int64_t numel, 
   const int64_t t = 4 * nq + g;
   if (t < numel) out[t] = n.template at<P>(t, rows);
 }
-// tie's vector body: both children evaluated on a quad before either output
-// quad is stored
-template <class P, class T, class N1, class N2>
-__global__ __launch_bounds__(256) void k_tie4(N1 n1, N2 n2, T* o1, T* o2, int64_t numel, int64_t rows) {
+// tie's loop nests: at every element (every quad in the vector body) all
+// children are evaluated before any output is stored, so an output may be
+// an input of another child (core/functions/expr/tie.hpp:80-115 reads the
+// whole fusion sequence, then stores it).
+template <class... Ns> struct nodes;
+template <> struct nodes<> {
+  bool aligned() const { return true; }
+  bool bcast_reads(const void*) const { return false; }
+};
+template <class H, class... R> struct nodes<H, R...> {
+  H h;
+  nodes<R...> t;
+  bool aligned() const { return h.aligned() && t.aligned(); }
+  bool bcast_reads(const void* p) const { return h.bcast_reads(p) || t.bcast_reads(p); }
+};
+template <class T, int N> struct out_ptrs {
+  T* p[N];
+};
+template <class P, class T> NT2H_FD void eval_all(const nodes<>&, int64_t, int64_t, T*) {}
+template <class P, class T, class H, class... R>
+NT2H_FD void eval_all(const nodes<H, R...>& ns, int64_t i, int64_t rows, T* v) {
+  v[0] = ns.h.template at<P>(i, rows);
+  eval_all<P, T>(ns.t, i, rows, v + 1);
+}
+template <class P, class T> NT2H_FD void eval_all4(const nodes<>&, int64_t, int64_t, quad<T>*) {}
+template <class P, class T, class H, class... R>
+NT2H_FD void eval_all4(const nodes<H, R...>& ns, int64_t i, int64_t rows, quad<T>* q) {
+  q[0] = ns.h.template at4<P>(i, rows);
+  eval_all4<P, T>(ns.t, i, rows, q + 1);
+}
+template <class P, class T, int N, class Ns>
+__global__ __launch_bounds__(256) void k_tie4(Ns ns, out_ptrs<T, N> o, int64_t numel, int64_t rows) {
   using V = typename nt2h::Vec16<T>::type;
   constexpr int W = nt2h::Vec16<T>::n;
   const int64_t nq = numel / 4;
   const int64_t stride = (int64_t)gridDim.x * 256;
   const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
   for (int64_t q = g; q < nq; q += stride) {
-    const quad<T> a = n1.template at4<P>(4 * q, rows), b = n2.template at4<P>(4 * q, rows);
+    quad<T> v[N];
+    eval_all4<P, T>(ns, 4 * q, rows, v);
 #pragma unroll
-    for (int h = 0; h < 4 / W; ++h) {
-      V va, vb;
+    for (int j = 0; j < N; ++j) {
 #pragma unroll
-      for (int k = 0; k < W; ++k) {
-        va[k] = a.v[h * W + k];
-        vb[k] = b.v[h * W + k];
+      for (int h = 0; h < 4 / W; ++h) {
+        V x;
+#pragma unroll
+        for (int k = 0; k < W; ++k) x[k] = v[j].v[h * W + k];
+        __builtin_nontemporal_store(x, reinterpret_cast<V*>(o.p[j] + 4 * q) + h);
       }
-      __builtin_nontemporal_store(va, reinterpret_cast<V*>(o1 + 4 * q) + h);
-      __builtin_nontemporal_store(vb, reinterpret_cast<V*>(o2 + 4 * q) + h);
     }
   }
   const int64_t t = 4 * nq + g;
   if (t < numel) {
-    T a = n1.template at<P>(t, rows), b = n2.template at<P>(t, rows);
-    o1[t] = a;
-    o2[t] = b;
+    T v[N];
+    eval_all<P, T>(ns, t, rows, v);
+#pragma unroll
+    for (int j = 0; j < N; ++j) o.p[j][t] = v[j];
   }
 }
-template <class P, class T, class N1, class N2>
-__global__ __launch_bounds__(256) void k_tie(N1 n1, N2 n2, T* o1, T* o2, int64_t numel, int64_t rows) {
+template <class P, class T, int N, class Ns>
+__global__ __launch_bounds__(256) void k_tie(Ns ns, out_ptrs<T, N> o, int64_t numel, int64_t rows) {
   const int64_t stride = (int64_t)gridDim.x * 256;
   for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < numel; i += stride) {
-    T a = n1.template at<P>(i, rows), b = n2.template at<P>(i, rows);  // every child read before any output is written
-    o1[i] = a;
-    o2[i] = b;
+    T v[N];
+    eval_all<P, T>(ns, i, rows, v);
+#pragma unroll
+    for (int j = 0; j < N; ++j) o.p[j][i] = v[j];
   }
 }
 
@@ -790,66 +897,125 @@ NT2HIP_SCAN_FUNCTOR(cummin, NT2H_SCAN_CUMMIN)
 #undef NT2HIP_SCAN_FUNCTOR
 
 // =============================================================== tie
-template <class T> struct tie2 {
-  table<T>& a;
-  table<T>& b;
-  template <class N1, class N2> struct exprs {
-    expr<T, N1> e1;
-    expr<T, N2> e2;
-  };
-  template <class N1, class N2> void operator=(const exprs<N1, N2>& es) {
-    extent x = merge<T>(es.e1.ext, es.e2.ext);
-    // both children are read at every index before either output is written
-    // (core/functions/expr/tie.hpp:80-115): outputs may alias inputs
-    table<T> ta(x.rows, x.cols), tb(x.rows, x.cols);
-    if (x.numel() > 0) {
-      if (es.e1.n.aligned() && es.e2.n.aligned()) {  // the outputs are fresh tables: aligned
-        int64_t nb = (x.numel() / 4 + 255) / 256;
+// tie(o1, ..., oN) = tie(e1, ..., eN): N outputs from one loop nest
+// (core/container/table/include/nt2/core/functions/expr/tie.hpp:23-160),
+// every child read at an element before any output is written there.
+template <class T, class... Ns> struct tie_exprs {
+  detail::nodes<Ns...> ns;
+  extent ext;
+};
+template <class T> struct bs_call_put {
+  const table<T> *S, *K, *T_, *r, *sig;
+  int precision;
+};
+namespace detail {
+inline detail::nodes<> make_nodes() { return {}; }
+template <class H, class... R> inline detail::nodes<H, R...> make_nodes(const H& h, const R&... r) {
+  return {h, make_nodes(r...)};
+}
+template <class T> inline extent merge_all(const extent& a) { return a; }
+template <class T, class... R> inline extent merge_all(const extent& a, const extent& b, const R&... r) {
+  return merge_all<T>(merge<T>(a, b), r...);
+}
+}  // namespace detail
+
+template <class T, int N> struct tie_t {
+  table<T>* o[N];
+
+  template <class... Ns> void operator=(const tie_exprs<T, Ns...>& es) {
+    static_assert(sizeof...(Ns) == N, "tie: as many expressions as outputs");
+    assign_all<exact>(es);
+  }
+  // tie(a, b) = tie(b, a): the right-hand tables as terminals
+  tie_t& operator=(const tie_t& rhs) {
+    assign_all<exact>(rhs.as_exprs(std::make_integer_sequence<int, N>()));
+    return *this;
+  }
+  template <int... I> auto as_exprs(std::integer_sequence<int, I...>) const {
+    return tie_exprs<T, decltype(as_expr(*o[I]).n)...>{detail::make_nodes(as_expr(*o[I]).n...),
+                                                       detail::merge_all<T>(o[I]->extent_()...)};
+  }
+  template <class P, class... Ns> void assign_all(const tie_exprs<T, Ns...>& es) {
+    extent x = es.ext;
+    x.row_bcast = false;
+    // an output that some child reads through sx() (at other elements) is
+    // written only after the whole loop nest: evaluate into fresh tables
+    bool stage = false;
+    for (int j = 0; j < N; ++j) stage = stage || es.ns.bcast_reads(o[j]->data());
+    table<T> tmp[N];
+    detail::out_ptrs<T, N> dst;
+    for (int j = 0; j < N; ++j) {
+      table<T>& t = stage ? tmp[j] : *o[j];
+      t.resize(x);
+      dst.p[j] = t.data();
+    }
+    const int64_t numel = x.numel();
+    if (numel > 0) {
+      bool al = es.ns.aligned();
+      for (int j = 0; j < N; ++j) al = al && (reinterpret_cast<uintptr_t>(dst.p[j]) & 15u) == 0;
+      using NS = detail::nodes<Ns...>;
+      if (al) {
+        int64_t nb = (numel / 4 + 255) / 256;
         if (nb < 1) nb = 1;
         if (nb > (int64_t(1) << 30)) nb = int64_t(1) << 30;
-        hipLaunchKernelGGL((detail::k_tie4<exact, T, N1, N2>), dim3((unsigned)nb), dim3(256), 0, current_stream(),
-                           es.e1.n, es.e2.n, ta.data(), tb.data(), x.numel(), x.rows);
+        hipLaunchKernelGGL((detail::k_tie4<P, T, N, NS>), dim3((unsigned)nb), dim3(256), 0, current_stream(), es.ns,
+                           dst, numel, x.rows);
       } else {
-        hipLaunchKernelGGL((detail::k_tie<exact, T, N1, N2>), dim3(detail::grid_for(x.numel())), dim3(256), 0,
-                           current_stream(), es.e1.n, es.e2.n, ta.data(), tb.data(), x.numel(), x.rows);
+        hipLaunchKernelGGL((detail::k_tie<P, T, N, NS>), dim3(detail::grid_for(numel)), dim3(256), 0,
+                           current_stream(), es.ns, dst, numel, x.rows);
       }
       hip_check(hipGetLastError(), "tie launch");
     }
-    a = std::move(ta);
-    b = std::move(tb);
+    if (stage)
+      for (int j = 0; j < N; ++j) *o[j] = std::move(tmp[j]);
   }
-  struct bs_call_put {
-    const table<T> *S, *K, *T_, *r, *sig;
-    int precision;
-  };
-  void operator=(const bs_call_put& q) {
+
+  // tie(call, put) = blackscholes(S, K, T, r, sig): nt2h_bs_fwd_f32/f64
+  void operator=(const bs_call_put<T>& q) {
+    static_assert(N == 2, "blackscholes yields two tables: tie(call, put)");
     extent x = q.S->extent_();
-    a.resize(x);
-    b.resize(x);
+    o[0]->resize(x);
+    o[1]->resize(x);
     int st = std::is_same<T, float>::value
                  ? nt2h_bs_fwd_f32(q.precision, (const float*)q.S->data(), (const float*)q.K->data(),
                                    (const float*)q.T_->data(), (const float*)q.r->data(), (const float*)q.sig->data(),
-                                   (float*)a.data(), (float*)b.data(), x.numel(), abi_stream())
+                                   (float*)o[0]->data(), (float*)o[1]->data(), x.numel(), abi_stream())
                  : nt2h_bs_fwd_f64(q.precision, (const double*)q.S->data(), (const double*)q.K->data(),
                                    (const double*)q.T_->data(), (const double*)q.r->data(),
-                                   (const double*)q.sig->data(), (double*)a.data(), (double*)b.data(), x.numel(),
-                                   abi_stream());
+                                   (const double*)q.sig->data(), (double*)o[0]->data(), (double*)o[1]->data(),
+                                   x.numel(), abi_stream());
     check(st, "nt2h_bs_fwd");
   }
 };
-template <class T> inline tie2<T> tie(table<T>& a, table<T>& b) { return {a, b}; }
-template <class A, class B> inline auto tie(const A& a, const B& b) {
+// assign<fused>(tie(...), tie(...)): the same loop nest with the FMA policy
+template <class P, class T, int N, class... Ns> inline void assign(tie_t<T, N>&& lhs, const tie_exprs<T, Ns...>& es) {
+  static_assert(sizeof...(Ns) == N, "tie: as many expressions as outputs");
+  lhs.template assign_all<P>(es);
+}
+
+// tie of writable tables: the left-hand side (or, on the right, their values)
+template <class T, class... R,
+          class = std::enable_if_t<(std::is_same<R, table<T>>::value && ...)>>
+inline tie_t<T, 1 + (int)sizeof...(R)> tie(table<T>& a, R&... rest) {
+  return {{&a, &rest...}};
+}
+// tie of expressions (tables, expressions and scalars mixed): the right-hand side
+template <class A, class... R,
+          class = std::enable_if_t<(is_table<A>::value || is_expr<A>::value) &&
+                                   ((is_table<R>::value || is_expr<R>::value) && ...)>,
+          class = void>
+inline auto tie(const A& a, const R&... rest) {
   using T = typename value_of<A>::type;
-  auto ea = as_expr(a);
-  auto eb = as_expr(b);
-  return typename tie2<T>::template exprs<decltype(ea.n), decltype(eb.n)>{ea, eb};
+  return tie_exprs<T, decltype(as_expr(a).n), decltype(as_expr(rest).n)...>{
+      detail::make_nodes(as_expr(a).n, as_expr(rest).n...),
+      detail::merge_all<T>(as_expr(a).ext, as_expr(rest).ext...)};
 }
 
 // Black-Scholes call and put of the five input tables (one launch):
 //   tie(call, put) = blackscholes(S, K, T, r, sig);
 template <class T>
-inline typename tie2<T>::bs_call_put blackscholes(const table<T>& S, const table<T>& K, const table<T>& Tm,
-                                                  const table<T>& r, const table<T>& sig, int precision = NT2H_EXACT) {
+inline bs_call_put<T> blackscholes(const table<T>& S, const table<T>& K, const table<T>& Tm, const table<T>& r,
+                                   const table<T>& sig, int precision = NT2H_EXACT) {
   extent x = S.extent_();
   if (!(K.extent_() == x && Tm.extent_() == x && r.extent_() == x && sig.extent_() == x))
     throw error("nt2hip::blackscholes: size mismatch", NT2H_ERR_INVALID_ARG);

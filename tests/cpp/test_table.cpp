@@ -22,6 +22,7 @@ void nt2ref_scan_lmh_f32(int op, const float* x, int64_t lo, int64_t mid, int64_
 }
 
 using nt2hip::table;
+namespace nt2 = nt2hip;  // NT2's demo bodies below compile as written
 static int failures = 0;
 static void report(const char* name, bool ok, const char* detail = "") {
   std::printf("%s %s %s\n", ok ? "PASS" : "FAIL", name, detail);
@@ -48,7 +49,213 @@ static std::vector<float> uniform(size_t n, float lo, float hi, uint64_t seed) {
   return v;
 }
 
+// ---------------------------------------------------------------------------
+// NT2's own demo expressions, bodies copied unchanged (nt2:: is nt2hip::).
+// demo/blackscholes/nt2/blackandscholes_nt2.cpp:53-60
+template <typename T> struct blackandscholes_demo {
+  nt2::table<T> Sa, Xa, Ta, ra, va, R;
+  void operator()() {
+    nt2::table<T> da = nt2::sqrt(Ta);
+    nt2::table<T> d1 = nt2::log(Sa/Xa) + (nt2::fma(nt2::sqr(va),T(0.5),ra)*Ta)/(va*da);
+    nt2::table<T> d2 = nt2::fnms(va,da,d1);
+
+    R = nt2::fnms(Xa*nt2::exp(-ra*Ta),nt2::fastnormcdf(d2),Sa*nt2::fastnormcdf(d1));
+  }
+};
+// demo/blackscholes/nt2_merged/blackandscholes_nt2_merged.cpp:57-64
+template <typename T> struct blackandscholes_merged_demo {
+  nt2::table<T> Sa, Xa, Ta, ra, va, R, da, d1, d2;
+  void operator()() {
+    nt2::tie(da,d1,d2,R) = nt2::tie ( nt2::sqrt(Ta)
+                                    , nt2::log(Sa/Xa) + (nt2::fma(nt2::sqr(va),T(0.5),ra)*Ta)/(va*da)
+                                    , nt2::fnms(va,da,d1)
+                                    , nt2::fnms(Xa*nt2::exp(-ra*Ta),nt2::fastnormcdf(d2),Sa*nt2::fastnormcdf(d1))
+                                    );
+  }
+};
+// demo/saxpy/nt2/saxpy_nt2.cpp:39-42
+template <typename T> struct axpy_demo {
+  T alpha;
+  nt2::table<T> X, Y;
+  void operator()() { Y = alpha * X + Y; }
+};
+// demo/sdot/nt2/sdot_nt2.cpp:35-38
+template <typename T> struct dot_demo {
+  nt2::table<T> X, Y;
+  T res_;
+  void operator()() { res_ = nt2::globalsum(X*Y); }
+};
+
+// The oracle's composition of the same functors (NT2's scalar path: every
+// operation rounded, fma/fnms as a product then a sum, each functor the
+// oracle's restatement): the demo's four loop nests per element, with the
+// merged demo's tie reading da, d1 and d2 as they were before the call.
+struct bs_terms {
+  std::vector<float> da, d1, d2, R;
+};
+static std::vector<float> map_ref(int op, const std::vector<float>& x) {
+  std::vector<float> y(x.size());
+  nt2ref_map_f32(op, x.data(), y.data(), (int64_t)x.size());
+  return y;
+}
+static bs_terms bs_demo_ref(const std::vector<float>& S, const std::vector<float>& X, const std::vector<float>& T,
+                            const std::vector<float>& r, const std::vector<float>& v, const std::vector<float>* da0,
+                            const std::vector<float>* d10, const std::vector<float>* d20) {
+  const size_t n = S.size();
+  bs_terms o;
+  o.da = map_ref(2, T);  // sqrt
+  const std::vector<float>& da = da0 ? *da0 : o.da;
+  std::vector<float> q(n), mrt(n);
+  for (size_t i = 0; i < n; ++i) { q[i] = S[i] / X[i]; mrt[i] = -r[i] * T[i]; }
+  std::vector<float> lq = map_ref(1, q), e = map_ref(0, mrt);
+  o.d1.resize(n);
+  o.d2.resize(n);
+  for (size_t i = 0; i < n; ++i) {
+    float sq = v[i] * v[i];
+    float f = sq * 0.5f;
+    f = f + r[i];  // fma(sqr(v), 0.5, r): two roundings
+    float num = f * T[i];
+    float den = v[i] * da[i];
+    float t = num / den;
+    o.d1[i] = lq[i] + t;
+  }
+  const std::vector<float>& d1 = d10 ? *d10 : o.d1;
+  for (size_t i = 0; i < n; ++i) {
+    float p = v[i] * da[i];
+    o.d2[i] = d1[i] - p;  // fnms(v, da, d1) = d1 - v*da
+  }
+  const std::vector<float>& d2 = d20 ? *d20 : o.d2;
+  std::vector<float> n1 = map_ref(10, d1), n2 = map_ref(10, d2);  // fastnormcdf
+  o.R.resize(n);
+  for (size_t i = 0; i < n; ++i) {
+    float a = X[i] * e[i];
+    float b = S[i] * n1[i];
+    float p = a * n2[i];
+    o.R[i] = b - p;  // fnms(a, n2, b)
+  }
+  return o;
+}
+
+static void demo_checks() {
+  {  // the demo's own inputs (Sa(i) = ... = T(i), i = 1..4096) and C2-like ones
+    for (int which = 0; which < 2; ++which) {
+      const size_t n = which == 0 ? 4096 : 100003;
+      std::vector<float> S(n), X(n), T(n), r(n), v(n);
+      if (which == 0) {
+        for (size_t i = 1; i <= n; ++i) S[i - 1] = X[i - 1] = T[i - 1] = r[i - 1] = v[i - 1] = float(i);
+      } else {
+        S = uniform(n, 5.f, 30.f, 21); X = uniform(n, 1.f, 100.f, 22); T = uniform(n, 0.25f, 10.f, 23);
+        r.assign(n, 0.02f); v.assign(n, 0.30f);
+      }
+      blackandscholes_demo<float> d{table<float>(S), table<float>(X), table<float>(T), table<float>(r),
+                                    table<float>(v), table<float>()};
+      d();
+      bs_terms want = bs_demo_ref(S, X, T, r, v, nullptr, nullptr, nullptr);
+      report(which == 0 ? "demo_blackscholes_nt2_demo_inputs_bit_exact" : "demo_blackscholes_nt2_bit_exact",
+             same_bits(d.R.to_host(), want.R));
+    }
+  }
+  {  // merged demo: one loop nest, four outputs; the tie reads da, d1, d2 as
+     // they were before the call (NT2's semantics), so after four calls from
+     // arbitrary values R equals the four-loop demo's R bit for bit
+    const size_t n = 100003;
+    std::vector<float> S = uniform(n, 5.f, 30.f, 31), X = uniform(n, 1.f, 100.f, 32), T = uniform(n, 0.25f, 10.f, 33);
+    std::vector<float> r(n, 0.02f), v(n, 0.30f);
+    std::vector<float> da0 = uniform(n, 0.5f, 3.f, 34), d10 = uniform(n, -2.f, 2.f, 35), d20 = uniform(n, -2.f, 2.f, 36);
+    blackandscholes_merged_demo<float> m;
+    m.Sa = table<float>(S); m.Xa = table<float>(X); m.Ta = table<float>(T); m.ra = table<float>(r);
+    m.va = table<float>(v); m.R = table<float>(n); m.da = table<float>(da0); m.d1 = table<float>(d10);
+    m.d2 = table<float>(d20);
+    bool ok = true;
+    std::vector<float> a = da0, b = d10, c = d20;
+    for (int call = 0; call < 4; ++call) {
+      m();
+      bs_terms want = bs_demo_ref(S, X, T, r, v, &a, &b, &c);
+      ok = ok && same_bits(m.da.to_host(), want.da) && same_bits(m.d1.to_host(), want.d1) &&
+           same_bits(m.d2.to_host(), want.d2) && same_bits(m.R.to_host(), want.R);
+      a = want.da; b = want.d1; c = want.d2;
+    }
+    report("demo_blackscholes_merged_tie4_stale_reads_bit_exact", ok);
+    bs_terms plain = bs_demo_ref(S, X, T, r, v, nullptr, nullptr, nullptr);
+    report("demo_blackscholes_merged_converges_to_demo", same_bits(m.R.to_host(), plain.R));
+  }
+  {  // saxpy demo: X(i) = Y(i) = T(i-1); known answer (alpha+1)(i-1), bit-exact vs the oracle
+    const size_t n = size_t(1) << 20;
+    std::vector<float> x(n);
+    for (size_t i = 1; i <= n; ++i) x[i - 1] = float(i - 1);
+    axpy_demo<float> d{2.5f, table<float>(x), table<float>(x)};
+    d();
+    std::vector<float> want = x;
+    nt2ref_saxpy_f32(2.5f, x.data(), want.data(), (int64_t)n);
+    auto got = d.Y.to_host();
+    bool known = true;
+    for (size_t i = 0; i < n && known; ++i) known = got[i] == 3.5f * float(i);
+    report("demo_saxpy_bit_exact_and_known_answer", same_bits(got, want) && known);
+  }
+  {  // sdot demo: X(i) = Y(i) = T(i-1); below 4096 the products and the closed
+     // form sum (n-1) n (2n-1) / 6 are exact, so the result is its rounding
+    for (size_t n : {size_t(4096), size_t(1) << 20}) {
+      std::vector<float> x(n);
+      for (size_t i = 1; i <= n; ++i) x[i - 1] = float(i - 1);
+      dot_demo<float> d{table<float>(x), table<float>(x), 0.f};
+      d();
+      double ex[2];
+      nt2ref_exact_dot_f32(x.data(), x.data(), (int64_t)n, ex);
+      float want = (float)(ex[0] + ex[1]);
+      bool ok = std::fabs(d.res_ - want) <= std::nextafter(want, INFINITY) - want;
+      if (n == 4096) {
+        double m = (double)n;
+        ok = ok && d.res_ == (float)((m - 1) * m * (2 * m - 1) / 6);
+      }
+      report(n == 4096 ? "demo_sdot_closed_form" : "demo_sdot_within_1ulp", ok);
+    }
+  }
+  {  // fma / fms / fnms / fnma nodes: two roundings under exact, FMA under fused
+    const int64_t n = 65536 + 3;
+    auto ha = uniform(n, -3.f, 3.f, 41), hb = uniform(n, -3.f, 3.f, 42), hc = uniform(n, -3.f, 3.f, 43);
+    table<float> A(ha), B(hb), C(hc), Z;
+    Z = nt2::fma(A, B, C) - nt2::fms(A, 2.0f, C) + nt2::fnms(A, B, C) * nt2::fnma(1.5f, B, C);
+    std::vector<float> want(n), fz(n);
+    for (int64_t i = 0; i < n; ++i) {
+      float p = ha[i] * hb[i], f1 = p + hc[i];
+      float q = ha[i] * 2.0f, f2 = q - hc[i];
+      float f3 = hc[i] - p;
+      float u = 1.5f * hb[i], f4 = -hc[i] - u;
+      float s1 = f1 - f2, s2 = f3 * f4;
+      want[i] = s1 + s2;
+    }
+    report("fma_fms_fnms_fnma_exact_bit_exact", same_bits(Z.to_host(), want));
+    nt2::assign<nt2hip::fused>(Z, nt2::fma(A, B, C));
+    auto g = Z.to_host();
+    bool ok = true;
+    for (int64_t i = 0; i < n && ok; ++i) ok = g[i] == std::fmaf(ha[i], hb[i], hc[i]);
+    report("fma_fused_single_rounding", ok);
+  }
+  {  // tie of tables (a swap), a tie whose output is read through sx(), and the fused tie
+    const int64_t rows = 33, cols = 9;
+    auto h1 = uniform(rows * cols, -1.f, 1.f, 51), h2 = uniform(rows * cols, -1.f, 1.f, 52), h3 = uniform(cols, 1.f, 2.f, 53);
+    table<float> P(h1, rows, cols), Q(h2, rows, cols), W(h3, 1, cols), M(h1, rows, cols);
+    nt2::tie(P, Q) = nt2::tie(Q, P);
+    bool ok = same_bits(P.to_host(), h2) && same_bits(Q.to_host(), h1);
+    // W is both an output and the sx() row of a child: staged, so every
+    // element reads the row as it was before the call
+    nt2::tie(M, W) = nt2::tie(M * nt2::sx(W), M + 1.0f);
+    auto hm = M.to_host(), hw = W.to_host();
+    ok = ok && W.rows() == rows && W.cols() == cols;
+    for (int64_t i = 0; i < rows * cols && ok; ++i) ok = hm[i] == h1[i] * h3[i / rows] && hw[i] == h1[i] + 1.0f;
+    report("tie_tables_swap_and_staged_sx_output", ok);
+    table<float> A(h1, rows, cols), B(h2, rows, cols), C, D, E;
+    nt2::assign<nt2hip::fused>(nt2::tie(C, D, E), nt2::tie(nt2::fma(A, B, 1.0f), A - B, nt2::sqrt(nt2::abs(A))));
+    auto hc = C.to_host(), hd = D.to_host(), he = E.to_host();
+    bool ok2 = true;
+    for (int64_t i = 0; i < rows * cols && ok2; ++i)
+      ok2 = hc[i] == std::fmaf(h1[i], h2[i], 1.0f) && hd[i] == h1[i] - h2[i] && he[i] == std::sqrt(std::fabs(h1[i]));
+    report("tie3_fused", ok2);
+  }
+}
+
 int main() {
+  demo_checks();
   const int64_t n = (1 << 20) + 7;
   auto hx = uniform(n, -1.f, 1.f, 1), hy = uniform(n, -1.f, 1.f, 2);
   table<float> X(hx), Y(hy);
