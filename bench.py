@@ -440,17 +440,17 @@ def report_host(dev, stream, n=1 << 24):
 
 
 def report_colfold(dev, stream, rows=16384, cols=8192):
-    """Column fold + adjoint over 16384 x 8192: time, and error vs a torch FP64 evaluation on 64 columns."""
+    """Column fold + adjoint over 16384 x 8192, FAST and FUSED arithmetic: time,
+    and error vs a torch FP64 evaluation on 64 columns (over each output's sum of
+    |terms|)."""
     import torch
 
+    import nt2_amd
     from nt2_amd import ops, synth
 
     X_np, s_np, k_np, p_np = synth.colfold_inputs(rows, cols)
     X = torch.from_numpy(X_np).to(dev)
     s, k, p = (torch.from_numpy(a).to(dev) for a in (s_np, k_np, p_np))
-    z, ds, dk, dp = ops.colfold_gc(X, s, k, p)
-    t = _events(stream, lambda: ops.colfold_gc(X, s, k, p), 5)
-    tf = _events(stream, lambda: ops.colfold_gc(X, s, k, p, grad=False), 5)
     sel = torch.arange(0, cols, cols // 64, device=dev)
     Xd, sd, kd, pd = X[sel].double(), s[sel].double()[:, None], k[sel].double()[:, None], p[sel].double()[:, None]
     e = torch.exp(-Xd * Xd / (2 * sd * sd))
@@ -458,11 +458,18 @@ def report_colfold(dev, stream, rows=16384, cols=8192):
     zc, sn = e * torch.cos(arg), e * torch.sin(arg)
     ref = [zc.sum(1), (zc * Xd * Xd).sum(1) / sd[:, 0] ** 3, -(sn * Xd).sum(1), -sn.sum(1)]
     mag = [e.sum(1), (e * Xd * Xd).sum(1) / sd[:, 0] ** 3, (e * Xd.abs()).sum(1), e.sum(1)]
-    errs = [float(((g[sel].double() - r).abs() / m).max()) for g, r, m in zip((z, ds, dk, dp), ref, mag)]
-    return [{"report": "colfold", "rows": rows, "cols": cols, "fwd_adj_us": t * 1e6, "fwd_only_us": tf * 1e6,
-             "fwd_adj_gbps": 4 * rows * cols / t / 1e9, "hbm_frac": 4 * rows * cols / t / 8e12,
-             "elements_per_s": rows * cols / t,
-             "err_vs_fp64_over_sum_abs_terms": dict(zip(("z", "dz_ds", "dz_dk", "dz_dp"), errs))}]
+    recs = []
+    for name, prec in (("fast", nt2_amd.FAST), ("fused", nt2_amd.FUSED)):
+        z, ds, dk, dp = ops.colfold_gc(X, s, k, p, precision=prec)
+        t = _events(stream, lambda: ops.colfold_gc(X, s, k, p, precision=prec), 20)
+        tf = _events(stream, lambda: ops.colfold_gc(X, s, k, p, grad=False, precision=prec), 20)
+        errs = [float(((g[sel].double() - r).abs() / m).max()) for g, r, m in zip((z, ds, dk, dp), ref, mag)]
+        recs.append({"report": "colfold", "precision": name, "rows": rows, "cols": cols,
+                     "fwd_adj_us": t * 1e6, "fwd_only_us": tf * 1e6,
+                     "fwd_adj_gbps": 4 * rows * cols / t / 1e9, "hbm_frac": 4 * rows * cols / t / 8e12,
+                     "fwd_only_hbm_frac": 4 * rows * cols / tf / 8e12, "elements_per_s": rows * cols / t,
+                     "err_vs_fp64_over_sum_abs_terms": dict(zip(("z", "dz_ds", "dz_dk", "dz_dp"), errs))})
+    return recs
 
 
 def main():
@@ -583,6 +590,12 @@ def main():
     big = p64.abs() >= 1e-3 * S64
     acc["price_max_ulp_where_price>=1e-3*S"] = float(price_ulp[big].max())
     acc["totals_rel_err"] = [float(abs(totals32[i] - totals64[i]) / max(abs(totals64[i]), 1e-300)) for i in range(2)]
+    # the stated bounds (nt2_amd.accuracy), evaluated on this run's outputs
+    from nt2_amd import accuracy
+
+    m = accuracy.metrics(tuple(g64[:6]), f32_out, outs64, totals32, totals64)
+    bound = {"measured": m, "bound": accuracy.BOUNDS, "violations": accuracy.check(m)}
+    bound["pass"] = not bound["violations"]
 
     # device copy of the same byte count (read 24 B + write 24 B per option)
     src = torch.empty(n * 6, dtype=torch.float32, device=dev)
@@ -660,6 +673,7 @@ def main():
             "fp32_speedup": t64 / avg_kern_s,
         },
         "accuracy_vs_fp64": acc,
+        "accuracy_bound": bound,
     }
     if not args.no_cpu:
         line["cpu_baseline"] = _cpu_baseline(host)

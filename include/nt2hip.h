@@ -71,10 +71,22 @@ typedef enum nt2h_op {
   NT2H_OP_EXP10 = 12       /* nt2::exp10, float only                                              */
 } nt2h_op;
 
-/* Arithmetic of the map kernels: EXACT reproduces NT2's scalar functor
- * bit for bit (two-rounding fma/horner, IEEE division); FUSED uses FMA and
- * a refined hardware reciprocal (within a stated ULP bound of EXACT). */
-typedef enum nt2h_precision { NT2H_EXACT = 0, NT2H_FUSED = 1 } nt2h_precision;
+/* Arithmetic of the kernels.
+ *  EXACT  NT2's scalar functor (two-rounding fma/horner, IEEE division):
+ *         bit-identical to NT2's scalar path for every functor and both
+ *         types, except float sin/cos of |x| > 2^18 pi, where NT2 runs fdlibm's
+ *         float Payne-Hanek and this library the correctly rounded reduction
+ *         (at most 2 ULP from NT2's result there, at most 1 ULP from the
+ *         correctly rounded sin/cos), and double sin/cos above 2^20 pi/2
+ *         (at most 2 ULP from NT2's result).
+ *  FUSED  NT2's algorithms and coefficients with FMA and a refined hardware
+ *         reciprocal (within a stated ULP bound of EXACT).
+ *  FAST   column fold only (nt2h_colfold_gc_f32_prec): hardware exp2
+ *         (v_exp_f32) and a reduction of the argument prescaled by 1/pi;
+ *         each term's error is within a few 2^-24 of |term| plus 2^-24 of
+ *         its argument, stated against the FP64 fold as a fraction of the
+ *         column's sum of |terms|. */
+typedef enum nt2h_precision { NT2H_EXACT = 0, NT2H_FUSED = 1, NT2H_FAST = 2 } nt2h_precision;
 
 /* Summation order of the reductions. */
 typedef enum nt2h_sum_mode {
@@ -261,6 +273,15 @@ int nt2h_bs_fwd_adj_f64(const double* S, const double* K, const double* T, const
 int nt2h_colfold_gc_f32(const float* X, int64_t rows, int64_t cols, int64_t ld, const float* s_,
                         const float* k_, const float* p_, const float* gz, float* z, float* dz_ds,
                         float* dz_dk, float* dz_dp, nt2h_stream_t s);
+/* The same with the arithmetic chosen: NT2H_FAST (what nt2h_colfold_gc_f32
+ * runs: exp(-x^2/(2 s^2)) = v_exp_f32(x^2 * (-log2(e)/(2 s^2))), and
+ * sin/cos(k x + p) = +-sin/cos(pi f) with f = y - rint(y), y = (k/pi) x + p/pi)
+ * or NT2H_FUSED (NT2's exp and sin/cos algorithms with FMA).  Stated bounds
+ * against the FP64 fold, each as a fraction of the column's sum of |terms|
+ * of that output: FAST and FUSED both <= 1e-6 at 16384 x 8192 (tested). */
+int nt2h_colfold_gc_f32_prec(int precision, const float* X, int64_t rows, int64_t cols, int64_t ld,
+                             const float* s_, const float* k_, const float* p_, const float* gz, float* z,
+                             float* dz_ds, float* dz_dk, float* dz_dp, nt2h_stream_t s);
 
 #ifdef __cplusplus
 }

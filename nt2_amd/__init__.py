@@ -23,7 +23,7 @@ OPS = {
     "sin": 7, "cos": 8, "normpdf": 9, "fastnormcdf": 10, "exp2": 11, "exp10": 12,
 }
 F64_OPS = {k: v for k, v in OPS.items() if v <= 9}
-EXACT, FUSED = 0, 1
+EXACT, FUSED, FAST = 0, 1, 2
 SUM_NAIVE, SUM_KAHAN, SUM_PAIRWISE = 0, 1, 2
 H2H, H2D, D2H, D2D, DEFAULT = 0, 1, 2, 3, 4
 FOLD_OPS = {"sum": 0, "prod": 1, "maximum": 2, "minimum": 3, "asum1": 4, "asum2": 5, "norm2": 6, "mean": 7}
@@ -61,6 +61,7 @@ SIGNATURES = {
     "nt2h_bs_fwd_adj_f64": (_I, [_P] * 13 + [_I64, _P]),
     "nt2h_bs_fwd_adj_f32_noalloc": (_I, [_P] * 13 + [_I64, _P]),
     "nt2h_colfold_gc_f32": (_I, [_P, _I64, _I64, _I64] + [_P] * 8 + [_P]),
+    "nt2h_colfold_gc_f32_prec": (_I, [_I, _P, _I64, _I64, _I64] + [_P] * 8 + [_P]),
     "nt2h_fold_f32": (_I, [_I, _P, _I64, _I64, _I64, _P, _P]),
     "nt2h_fold_f64": (_I, [_I, _P, _I64, _I64, _I64, _P, _P]),
     "nt2h_scan_f64": (_I, [_I, _P, _I64, _I64, _I64, _P, _P]),

@@ -10,7 +10,7 @@ from __future__ import annotations
 
 import ctypes as C
 
-from . import (DEFAULT, EXACT, F64_OPS, FOLD_OPS, FUSED, OPS, SCAN_OPS, SUM_KAHAN, call)
+from . import (DEFAULT, EXACT, F64_OPS, FAST, FOLD_OPS, FUSED, OPS, SCAN_OPS, SUM_KAHAN, call)
 from .layout import streams
 
 
@@ -137,10 +137,12 @@ def bs_fwd_adj(S, K, T, r, sig, w, totals: bool = True, outs=None, tot_out=None)
     return (*outs, tot)
 
 
-def colfold_gc(X, s, k, p, gz=None, grad: bool = True):
+def colfold_gc(X, s, k, p, gz=None, grad: bool = True, precision: int = FAST):
     """X: (cols, rows) row-major CUDA tensor == column-major (rows, cols) table.
 
-    Returns z and, when grad, (dz_ds, dz_dk, dz_dp) for cotangent gz (None = ones)."""
+    Returns z and, when grad, (dz_ds, dz_dk, dz_dp) for cotangent gz (None = ones).
+    precision: FAST (hardware exp2, argument prescaled by 1/pi) or FUSED (NT2's
+    exp and sin/cos algorithms with FMA)."""
     torch = _torch()
     _chk(X, torch.float32, "X")
     cols, rows = X.shape
@@ -154,8 +156,8 @@ def colfold_gc(X, s, k, p, gz=None, grad: bool = True):
             raise ValueError(f"colfold_gc: gz must hold one value per column ({cols})")
     z = torch.empty(cols, dtype=torch.float32, device=X.device)
     g = [torch.empty_like(z) for _ in range(3)] if grad else [None, None, None]
-    call("nt2h_colfold_gc_f32", _ptr(X), rows, cols, rows, _ptr(s), _ptr(k), _ptr(p), _ptr(gz), _ptr(z),
-         *[_ptr(t) for t in g], _stream())
+    call("nt2h_colfold_gc_f32_prec", precision, _ptr(X), rows, cols, rows, _ptr(s), _ptr(k), _ptr(p), _ptr(gz),
+         _ptr(z), *[_ptr(t) for t in g], _stream())
     return (z, *g) if grad else z
 
 
