@@ -27,7 +27,15 @@
  *    replayed; they record kernels only (no memset or copy nodes).  While the
  *    stream is capturing, result pointers must be device memory, and the
  *    same call must have run once on that stream beforehand so that its
- *    workspace and scratch exist (allocation is not capturable).
+ *    workspace and scratch exist (allocation is not capturable).  A graph
+ *    uses the workspace of its capture stream: it must not run at the same
+ *    time as calls on that stream or as another graph recorded on it
+ *    (replays and calls in stream order are fine).  Growing a stream's
+ *    scratch never frees the buffer a graph recorded; nt2h_release_stream
+ *    does.
+ *  - Per-stream state (reduction workspace, scratch, look-back state) lives
+ *    until nt2h_release_stream(s), to be called before the stream is
+ *    destroyed.
  */
 #ifndef NT2HIP_H
 #define NT2HIP_H
@@ -50,8 +58,11 @@ typedef enum nt2h_status {
   NT2H_ERR_HIP = -5,          /* a HIP runtime call failed (see nt2h_last_hip_error) */
   NT2H_ERR_OUT_OF_MEMORY = -6,/* device allocation failed */
   NT2H_ERR_TOO_LARGE = -7,    /* size beyond what the entry point supports */
-  NT2H_ERR_CAPTURE = -8       /* the call needs a workspace allocation while its stream is capturing:
+  NT2H_ERR_CAPTURE = -8,      /* the call needs a workspace allocation while its stream is capturing:
                                  run it once on that stream before the capture */
+  NT2H_ERR_WAIT = -9          /* reported by nt2h_sync: a bounded wait inside a kernel on the stream
+                                 gave up (or a launch shared its state with another): that call's
+                                 results are invalid */
 } nt2h_status;
 
 /* Elementwise functor ids (the NT2 functors on the path). */
@@ -123,7 +134,27 @@ int nt2h_memcpy(void* dst, const void* src, size_t bytes, int kind, nt2h_stream_
  * the data. */
 int nt2h_upload(void* dst_device, const void* src_host, size_t bytes, nt2h_stream_t s);
 int nt2h_download(void* dst_host, const void* src_device, size_t bytes, nt2h_stream_t s);
+/* Wait for the stream; then report NT2H_ERR_WAIT once if a bounded wait in a
+ * kernel on it gave up since the last check (the look-back scans), and
+ * clear that flag. */
 int nt2h_sync(nt2h_stream_t s);
+/* Free the per-(current device, stream) state the library keeps for s: the
+ * reduction workspace, the scratch (with every buffer it outgrew) and the
+ * look-back state.  Waits for the stream first.  Call it when no graph
+ * recorded on s will run again, before destroying s; a later call on s (or
+ * on a new stream with the same handle) starts from fresh state. */
+int nt2h_release_stream(nt2h_stream_t s);
+/* Device bytes the library holds in per-stream state (all devices and
+ * streams: workspaces, scratch with the buffers it outgrew, look-back
+ * state), for accounting and leak checks. */
+int nt2h_state_bytes(size_t* bytes);
+/* Test control of the look-back scans' bounded waits, for tests of the error
+ * path only: spin_cap polls before a waiting tile gives up (0 = the
+ * default, 2^20), and never_ready != 0 makes every predecessor's state read
+ * as not yet published, so that each waiting tile gives up after spin_cap
+ * polls (every kernel still ends; nt2h_sync then reports NT2H_ERR_WAIT).
+ * Affects later launches in this process. */
+int nt2h_test_set_lookback_wait(unsigned spin_cap, int never_ready);
 
 /* ---------------------------------------------------- elementwise (transform_)
  * NT2 transform: core/functions/transform.hpp:55-69 -> elementwise run_assign

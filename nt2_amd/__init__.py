@@ -32,7 +32,7 @@ SCAN_OPS = {"cumsum": 0, "cumprod": 1, "cummax": 2, "cummin": 3}
 STATUS = {
     0: "NT2H_OK", -1: "NT2H_ERR_INVALID_ARG", -2: "NT2H_ERR_NULL_POINTER", -3: "NT2H_ERR_BAD_OP",
     -4: "NT2H_ERR_NO_DEVICE", -5: "NT2H_ERR_HIP", -6: "NT2H_ERR_OUT_OF_MEMORY", -7: "NT2H_ERR_TOO_LARGE",
-    -8: "NT2H_ERR_CAPTURE",
+    -8: "NT2H_ERR_CAPTURE", -9: "NT2H_ERR_WAIT",
 }
 
 # every exported entry point with its ctypes signature (restype, argtypes)
@@ -48,6 +48,9 @@ SIGNATURES = {
     "nt2h_host_free": (_I, [_P]),
     "nt2h_memcpy": (_I, [_P, _P, _SZ, _I, _P]),
     "nt2h_sync": (_I, [_P]),
+    "nt2h_release_stream": (_I, [_P]),
+    "nt2h_state_bytes": (_I, [_P]),
+    "nt2h_test_set_lookback_wait": (_I, [C.c_uint, _I]),
     "nt2h_saxpy_f32": (_I, [_F, _P, _P, _I64, _P]),
     "nt2h_map_f32": (_I, [_I, _I, _P, _P, _I64, _P]),
     "nt2h_map_f64": (_I, [_I, _I, _P, _P, _I64, _P]),

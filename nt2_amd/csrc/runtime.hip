@@ -10,6 +10,7 @@
 #include <map>
 #include <mutex>
 #include <utility>
+#include <vector>
 
 #include "nt2h_common.hpp"
 
@@ -45,6 +46,12 @@ static bool capturing(nt2h_stream_t s) {
   return hipStreamIsCapturing(as_hip(s), &cs) == hipSuccess && cs != hipStreamCaptureStatusNone;
 }
 
+// Device bytes of one reduction workspace (the allocations below).
+static size_t workspace_bytes() {
+  return (size_t)kMaxPartials * kMaxFoldBlocks * sizeof(double) * 2 + (size_t)kMaxPartials * kMaxGroups * sizeof(double) +
+         (size_t)(kMaxGroups + 1) * kCounterStride * sizeof(unsigned) + kMaxPartials * sizeof(double);
+}
+
 int get_workspace(nt2h_stream_t s, Workspace* ws) {
   int dev = 0;
   NT2H_HIP(hipGetDevice(&dev));
@@ -70,71 +77,87 @@ int get_workspace(nt2h_stream_t s, Workspace* ws) {
 }
 
 namespace {
-struct Scratch {
+// A grow-only device buffer of one (device, stream).  Growing never frees
+// the buffer it replaces: a graph recorded on the stream keeps using the
+// old one, and kernels of earlier calls may still be running.  Retired
+// buffers are freed by nt2h_release_stream (at most twice the final size
+// in all, since the lookback state doubles and the scratch grows to fit).
+struct Grown {
   void* p = nullptr;
   size_t bytes = 0;
+  std::vector<std::pair<void*, size_t>> retired;
 };
 std::mutex g_scratch_mutex;
-std::map<std::pair<int, hipStream_t>, Scratch> g_scratch;
+std::map<std::pair<int, hipStream_t>, Grown> g_scratch;
+std::mutex g_lb_mutex;
+std::map<std::pair<int, hipStream_t>, Grown> g_lb;
 }  // namespace
 
 // Grow-only device scratch of one (device, stream) for multi-launch folds
 // and scans (segment summaries).  Calls on one stream are ordered, so a
-// buffer is never in use by two calls at once; growing synchronises the
-// stream before freeing the old buffer.
+// buffer is never in use by two calls at once.
 int get_scratch(nt2h_stream_t s, size_t bytes, void** out) {
   int dev = 0;
   NT2H_HIP(hipGetDevice(&dev));
   std::lock_guard<std::mutex> lock(g_scratch_mutex);
-  Scratch& sc = g_scratch[std::make_pair(dev, as_hip(s))];
+  Grown& sc = g_scratch[std::make_pair(dev, as_hip(s))];
   if (sc.bytes < bytes) {
     if (capturing(s)) return NT2H_ERR_CAPTURE;
-    if (sc.p) {
-      NT2H_HIP(hipStreamSynchronize(as_hip(s)));
-      NT2H_HIP(hipFree(sc.p));
-      sc.p = nullptr;
-      sc.bytes = 0;
-    }
     size_t want = bytes < (size_t(1) << 20) ? (size_t(1) << 20) : bytes;
-    NT2H_HIP(hipMalloc(&sc.p, want));
+    void* p = nullptr;
+    NT2H_HIP(hipMalloc(&p, want));
+    if (sc.p) sc.retired.emplace_back(sc.p, sc.bytes);
+    sc.p = p;
     sc.bytes = want;
   }
   *out = sc.p;
   return NT2H_OK;
 }
 
-namespace {
-struct Lookback {
-  void* p = nullptr;
-  size_t bytes = 0;
-};
-std::mutex g_lb_mutex;
-std::map<std::pair<int, hipStream_t>, Lookback> g_lb;
-}  // namespace
-
 int get_lookback(nt2h_stream_t s, size_t bytes, void** out) {
   int dev = 0;
   NT2H_HIP(hipGetDevice(&dev));
   std::lock_guard<std::mutex> lock(g_lb_mutex);
-  Lookback& lb = g_lb[std::make_pair(dev, as_hip(s))];
+  Grown& lb = g_lb[std::make_pair(dev, as_hip(s))];
   if (lb.bytes < bytes) {
     if (capturing(s)) return NT2H_ERR_CAPTURE;
-    if (lb.p) {
-      NT2H_HIP(hipStreamSynchronize(as_hip(s)));
-      NT2H_HIP(hipFree(lb.p));
-      lb.p = nullptr;
-      lb.bytes = 0;
-    }
     size_t want = (size_t)1 << 20;
     while (want < bytes) want <<= 1;
-    NT2H_HIP(hipMalloc(&lb.p, want));
+    void* p = nullptr;
+    NT2H_HIP(hipMalloc(&p, want));
     // zeroed once, synchronously: the kernels keep it consistent (epoch
     // tags on the tile states, counters reset by the last workgroup)
-    NT2H_HIP(hipMemset(lb.p, 0, want));
+    NT2H_HIP(hipMemset(p, 0, want));
+    if (lb.p) lb.retired.emplace_back(lb.p, lb.bytes);
+    lb.p = p;
     lb.bytes = want;
   }
   *out = lb.p;
   return NT2H_OK;
+}
+
+// Error flag of the look-back scans on this stream (ctrl[3] of its state:
+// 1 = a bounded wait gave up, 2 = more tickets than tiles): read and
+// cleared once the stream is idle.
+static int lookback_status(nt2h_stream_t s) {
+  int dev = 0;
+  NT2H_HIP(hipGetDevice(&dev));
+  void* p = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_lb_mutex);
+    auto it = g_lb.find(std::make_pair(dev, as_hip(s)));
+    if (it == g_lb.end() || !it->second.p) return NT2H_OK;
+    p = it->second.p;
+  }
+  unsigned flag = 0;
+  unsigned* f = static_cast<unsigned*>(p) + 3;
+  NT2H_HIP(hipMemcpyAsync(&flag, f, sizeof flag, hipMemcpyDeviceToHost, as_hip(s)));
+  NT2H_HIP(hipStreamSynchronize(as_hip(s)));
+  if (flag == 0) return NT2H_OK;
+  const unsigned zero = 0;
+  NT2H_HIP(hipMemcpyAsync(f, &zero, sizeof zero, hipMemcpyHostToDevice, as_hip(s)));
+  NT2H_HIP(hipStreamSynchronize(as_hip(s)));
+  return NT2H_ERR_WAIT;
 }
 
 int is_host_pointer(const void* p, bool* host, nt2h_stream_t s) {
@@ -207,6 +230,7 @@ const char* nt2h_strerror(int code) {
     case NT2H_ERR_OUT_OF_MEMORY: return "out of device memory";
     case NT2H_ERR_TOO_LARGE: return "size too large";
     case NT2H_ERR_CAPTURE: return "workspace allocation needed while the stream is capturing";
+    case NT2H_ERR_WAIT: return "a bounded wait in a kernel gave up: that call's results are invalid";
   }
   return "unknown status";
 }
@@ -282,7 +306,56 @@ int nt2h_memcpy(void* dst, const void* src, size_t bytes, int kind, nt2h_stream_
 
 int nt2h_sync(nt2h_stream_t s) {
   NT2H_HIP(hipStreamSynchronize(as_hip(s)));
+  return lookback_status(s);
+}
+
+int nt2h_state_bytes(size_t* bytes) {
+  if (!bytes) return NT2H_ERR_NULL_POINTER;
+  size_t n = 0;
+  {
+    std::lock_guard<std::mutex> lock(g_ws_mutex);
+    n += g_ws.size() * workspace_bytes();
+  }
+  for (auto* m : {&g_scratch, &g_lb}) {
+    std::lock_guard<std::mutex> lock(m == &g_scratch ? g_scratch_mutex : g_lb_mutex);
+    for (auto& kv : *m) {
+      n += kv.second.bytes;
+      for (auto& r : kv.second.retired) n += r.second;
+    }
+  }
+  *bytes = n;
   return NT2H_OK;
+}
+
+int nt2h_release_stream(nt2h_stream_t s) {
+  int dev = 0;
+  NT2H_HIP(hipGetDevice(&dev));
+  NT2H_HIP(hipStreamSynchronize(as_hip(s)));
+  const auto key = std::make_pair(dev, as_hip(s));
+  int st = NT2H_OK;
+  auto keep = [&](hipError_t e) {
+    if (e != hipSuccess && st == NT2H_OK) st = hip_check(e);
+  };
+  {
+    std::lock_guard<std::mutex> lock(g_ws_mutex);
+    auto it = g_ws.find(key);
+    if (it != g_ws.end()) {
+      Workspace& w = it->second;
+      for (void* p : {(void*)w.partials, (void*)w.gpartials, (void*)w.counters, (void*)w.result, (void*)w.bpartials})
+        if (p) keep(hipFree(p));
+      g_ws.erase(it);
+    }
+  }
+  for (auto* m : {&g_scratch, &g_lb}) {
+    std::lock_guard<std::mutex> lock(m == &g_scratch ? g_scratch_mutex : g_lb_mutex);
+    auto it = m->find(key);
+    if (it != m->end()) {
+      if (it->second.p) keep(hipFree(it->second.p));
+      for (auto& r : it->second.retired) keep(hipFree(r.first));
+      m->erase(it);
+    }
+  }
+  return st;
 }
 
 }  // extern "C"

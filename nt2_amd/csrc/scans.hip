@@ -44,6 +44,8 @@
 //    are reproducible.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include "nt2h_common.hpp"
 
 namespace nt2h {
@@ -678,8 +680,10 @@ __global__ __launch_bounds__(kPrefixBlock) void scan_prefix_kernel(int64_t segs,
 // Forward progress: a tile waits only on tiles with smaller tickets, which
 // are already running and publish their aggregate before waiting on
 // anything.  Every wait is nevertheless bounded (spin_cap polls with a
-// sleep between them): past it the workgroup records an error in ctrl[3]
-// and finishes, so the kernel always ends.
+// sleep between them): past it the workgroup publishes a poisoned state
+// (which every later tile of the run passes on instead of a value), records
+// an error in ctrl[3] and finishes, so the kernel always ends; nt2h_sync on
+// the stream reports and clears the error.
 // Sub-tiles of kBlock x W elements per look-back tile (the kernel's kLbSub,
 // chosen per call): 4 (16 KiB) for short jobs, 8 (32 KiB: half the tickets
 // and half the chain) once the 16 KiB
@@ -709,10 +713,19 @@ __device__ __forceinline__ void lb_state(unsigned long long* f, unsigned long lo
 __device__ __forceinline__ unsigned long long lb_state_get(const unsigned long long* f) {
   return __hip_atomic_load(const_cast<unsigned long long*>(f), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+// Wait settings of the look-back kernel (host side, read at each launch):
+// the poll limit and, for tests of the error path only, an epoch mask that
+// makes every predecessor's state read as "not yet", so that each waiting
+// tile gives up after spin_cap polls (see nt2h_test_set_lookback_wait).
+namespace {
+std::atomic<unsigned> g_lb_spin_cap{0};      // 0: kLbSpinCap
+std::atomic<unsigned> g_lb_epoch_mask{0};
+}  // namespace
 template <int OP, class T, int kLbSub>
 __global__ __launch_bounds__(kBlock) void scan_lookback_kernel(SGeo<T> g, int64_t tpr, int64_t ntiles,
                                                                unsigned* ctrl, unsigned long long* state,
-                                                               SSumm* agg, SSumm* incl, unsigned spin_cap) {
+                                                               SSumm* agg, SSumm* incl, unsigned spin_cap,
+                                                               unsigned epoch_mask) {
   using M = typename SFam<OP, T>::type;
   using V = typename M::V;
   using C = typename M::C;
@@ -734,6 +747,13 @@ __global__ __launch_bounds__(kBlock) void scan_lookback_kernel(SGeo<T> g, int64_
   __syncthreads();
   const int64_t t = sh_t[0];
   const unsigned epoch = sh_t[1];
+  if (t >= ntiles) {
+    // more tickets than tiles: another launch shares this state (a graph
+    // replayed alongside calls on its capture stream).  Touch nothing, flag
+    // the error for nt2h_sync.
+    if (threadIdx.x == 0) __hip_atomic_store(&ctrl[3], 2u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return;
+  }
   const int64_t o = t / tpr, i = t - o * tpr;  // run, tile within the run
   const int64_t m0 = i * kLbSub * kSub;
   const int64_t m1 = (m0 + kLbSub * kSub < g.mid) ? m0 + kLbSub * kSub : g.mid;
@@ -791,27 +811,34 @@ __global__ __launch_bounds__(kBlock) void scan_lookback_kernel(SGeo<T> g, int64_
         __builtin_amdgcn_s_waitcnt(0);
         lb_state(&state[t], tag | 1ull);
       }
-      int jl = -1;  // lane (= distance - 1) of the nearest inclusive predecessor
+      // nearest predecessor in a final state: inclusive (2), or poisoned (3)
+      // by a tile whose wait gave up — poison is passed on, so that no tile
+      // after a failed wait publishes a value as if it were valid
+      int jl = -1;     // lane (= distance - 1) of that predecessor
+      bool poison = false;
+      const unsigned want = epoch ^ epoch_mask;
       for (unsigned spins = 0;; ++spins) {
         const bool inrun = i - 1 - lane >= 0;
         const unsigned long long f = inrun ? lb_state_get(&state[t - 1 - lane]) : 0ull;
-        const bool ready = inrun && (f >> 32) == epoch && (f & 3ull) != 0;
-        const uint64_t mi = __ballot(ready && (f & 3ull) == 2ull), mr = __ballot(ready);
+        const bool ready = inrun && (unsigned)(f >> 32) == want && (f & 3ull) != 0;
+        const uint64_t mi = __ballot(ready && (f & 3ull) >= 2ull), mr = __ballot(ready);
+        const uint64_t mp = __ballot(ready && (f & 3ull) == 3ull);
         if (mi) {
           const int j = __builtin_ctzll(mi);
           const uint64_t need = (j == 0) ? 0ull : (~0ull >> (64 - j));  // lanes 0..j-1
           if ((mr & need) == need) {
             jl = j;
+            poison = ((mp >> j) & 1ull) != 0;
             break;
           }
         }
         if (spins >= spin_cap) {
-          if (lane == 0) __hip_atomic_store(&ctrl[3], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          poison = true;
           break;
         }
         __builtin_amdgcn_s_sleep(2);
       }
-      if (jl >= 0) {
+      if (!poison) {
         SSumm v{{0.0, 0.0}};
         if (lane == jl) v = lb_get(&incl[t - 1 - lane]);
         else if (lane < jl) v = lb_get(&agg[t - 1 - lane]);
@@ -819,9 +846,15 @@ __global__ __launch_bounds__(kBlock) void scan_lookback_kernel(SGeo<T> g, int64_
         for (int l = jl - 1; l >= 0; --l) P = M::ccomb(P, M::unpack(shfl_summ(v, l)));
       }
       if (lane == 0) {
-        lb_put(&incl[t], M::pack(M::ccomb(P, L[kLbSub])));
-        __builtin_amdgcn_s_waitcnt(0);
-        lb_state(&state[t], tag | 2ull);
+        if (poison) {
+          // the tile's output is invalid: flag the error for nt2h_sync
+          __hip_atomic_store(&ctrl[3], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          lb_state(&state[t], tag | 3ull);
+        } else {
+          lb_put(&incl[t], M::pack(M::ccomb(P, L[kLbSub])));
+          __builtin_amdgcn_s_waitcnt(0);
+          lb_state(&state[t], tag | 2ull);
+        }
       }
     }
     if (lane == 0) sh_p = M::pack(P);
@@ -899,12 +932,14 @@ int scan_launch(const T* x, T* y, int64_t lo, int64_t mid, int64_t hi, nt2h_stre
       unsigned long long* lbst = reinterpret_cast<unsigned long long*>(reinterpret_cast<char*>(base) + 256);
       SSumm* agg = reinterpret_cast<SSumm*>(lbst + ntiles);
       SSumm* incl = agg + ntiles;
+      const unsigned cap_set = g_lb_spin_cap.load(), cap = cap_set ? cap_set : kLbSpinCap;
+      const unsigned emask = g_lb_epoch_mask.load();
       if (sub == 8)
         hipLaunchKernelGGL((scan_lookback_kernel<OP, T, 8>), dim3((unsigned)ntiles), dim3(kBlock), 0, s, g, tpr,
-                           ntiles, ctrl, lbst, agg, incl, kLbSpinCap);
+                           ntiles, ctrl, lbst, agg, incl, cap, emask);
       else
         hipLaunchKernelGGL((scan_lookback_kernel<OP, T, 4>), dim3((unsigned)ntiles), dim3(kBlock), 0, s, g, tpr,
-                           ntiles, ctrl, lbst, agg, incl, kLbSpinCap);
+                           ntiles, ctrl, lbst, agg, incl, cap, emask);
       NT2H_HIP(hipGetLastError());
       return NT2H_OK;
     }
@@ -1017,6 +1052,12 @@ int nt2h_scan_f32(int op, const float* x, int64_t lo, int64_t mid, int64_t hi, f
 }
 int nt2h_scan_f64(int op, const double* x, int64_t lo, int64_t mid, int64_t hi, double* y, nt2h_stream_t s) {
   return scan_entry<double>(op, x, lo, mid, hi, y, s);
+}
+
+int nt2h_test_set_lookback_wait(unsigned spin_cap, int never_ready) {
+  g_lb_spin_cap.store(spin_cap);
+  g_lb_epoch_mask.store(never_ready ? 0x80000000u : 0u);
+  return NT2H_OK;
 }
 
 }  // extern "C"
