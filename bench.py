@@ -659,6 +659,9 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBPS,
             "traffic": traffic,
+            "traffic_source": "profile-derived, not measured in this run: rocprofv3 --pmc FETCH_SIZE and "
+                              "WRITE_SIZE of this kernel, scaled by the saxpy calibration "
+                              "(profiles/bs_fwd_adj_f32_2p24_pmc.txt)",
         },
         "kernel_us": avg_kern_s * 1e6,
         "kernel_us_median_of_50_event_pairs": med_us,
