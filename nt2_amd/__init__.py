@@ -14,7 +14,9 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "lib", "libnt2hip.so")
+# NT2H_LIB selects another build of the same sources (the sanitizer build of
+# the host code)
+LIB_PATH = os.environ.get("NT2H_LIB") or os.path.join(HERE, "lib", "libnt2hip.so")
 CSRC = os.path.join(HERE, "csrc")
 
 # enums of include/nt2hip.h

@@ -14,7 +14,8 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libnt2ref.so")
+# NT2REF_LIB selects another build of the same sources (the sanitizer build)
+LIB_PATH = os.environ.get("NT2REF_LIB") or os.path.join(HERE, "libnt2ref.so")
 
 # op ids (same numbering as NT2H_OP_* in include/nt2hip.h)
 OPS = {
