@@ -144,6 +144,14 @@ int nt2h_sync(nt2h_stream_t s);
  * recorded on s will run again, before destroying s; a later call on s (or
  * on a new stream with the same handle) starts from fresh state. */
 int nt2h_release_stream(nt2h_stream_t s);
+/* A per-stream buffer of the library, for code that launches the library's
+ * fold and scan kernels itself (the C++ front end instantiates them over
+ * expression trees, include/nt2hip/{fold,scan}_kernels.hpp): kind 0 is the
+ * stream's grow-only scratch, kind 1 its look-back state (zeroed when first
+ * allocated, then kept consistent by the kernels).  *out holds at least
+ * `bytes`; the buffer lives until nt2h_release_stream(s).  NT2H_ERR_CAPTURE
+ * if it would have to grow while s is capturing. */
+int nt2h_stream_buffer(nt2h_stream_t s, int kind, size_t bytes, void** out);
 /* Device bytes the library holds in per-stream state (all devices and
  * streams: workspaces, scratch with the buffers it outgrew, look-back
  * state), for accounting and leak checks. */

@@ -52,6 +52,7 @@ SIGNATURES = {
     "nt2h_sync": (_I, [_P]),
     "nt2h_release_stream": (_I, [_P]),
     "nt2h_state_bytes": (_I, [_P]),
+    "nt2h_stream_buffer": (_I, [_P, _I, _SZ, _P]),
     "nt2h_test_set_lookback_wait": (_I, [C.c_uint, _I]),
     "nt2h_saxpy_f32": (_I, [_F, _P, _P, _I64, _P]),
     "nt2h_map_f32": (_I, [_I, _I, _P, _P, _I64, _P]),

@@ -309,6 +309,14 @@ int nt2h_sync(nt2h_stream_t s) {
   return lookback_status(s);
 }
 
+int nt2h_stream_buffer(nt2h_stream_t s, int kind, size_t bytes, void** out) {
+  if (!out) return NT2H_ERR_NULL_POINTER;
+  *out = nullptr;
+  if (kind == 0) return get_scratch(s, bytes, out);
+  if (kind == 1) return get_lookback(s, bytes, out);
+  return NT2H_ERR_INVALID_ARG;
+}
+
 int nt2h_state_bytes(size_t* bytes) {
   if (!bytes) return NT2H_ERR_NULL_POINTER;
   size_t n = 0;
