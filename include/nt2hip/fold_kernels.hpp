@@ -53,13 +53,21 @@
 
 namespace nt2h {
 
-// Element source of a contiguous table.
+// Element source of a contiguous table.  The kernels read elements as
+// at(i), or as 16-byte vectors in two steps: raw(i) issues the loads of
+// elements i..i+W-1 (i a multiple of W), value(raw) does any arithmetic on
+// them — for a table none, for an expression tree evaluated per element its
+// nodes — so that a kernel can have several vectors' loads in flight before
+// the first one's arithmetic waits on memory; vec(i) is value(raw(i)).
 template <class T> struct PtrSrc {
+  using Raw = typename Vec16<T>::type;
   const T* x;
   __device__ __forceinline__ T at(int64_t i) const { return x[i]; }
-  __device__ __forceinline__ typename Vec16<T>::type vec(int64_t i) const {
-    return __builtin_nontemporal_load(reinterpret_cast<const typename Vec16<T>::type*>(x + i));
+  __device__ __forceinline__ Raw raw(int64_t i) const {
+    return __builtin_nontemporal_load(reinterpret_cast<const Raw*>(x + i));
   }
+  __device__ __forceinline__ typename Vec16<T>::type value(const Raw& r) const { return r; }
+  __device__ __forceinline__ typename Vec16<T>::type vec(int64_t i) const { return value(raw(i)); }
   bool aligned() const { return (reinterpret_cast<uintptr_t>(x) & 15u) == 0; }
 };
 
@@ -77,8 +85,8 @@ enum FoldOpId {
 
 // Table geometry: element m of output o = i + lo*h sits at i + lo*(m + mid*h);
 // the elements come from Src (a table, or an expression evaluated per
-// element: at(i) is element i, vec(i) the 16-byte vector of elements
-// i..i+W-1 for i a multiple of W).
+// element: at(i) is element i, raw(i)/value()/vec(i) the 16-byte vector of
+// elements i..i+W-1 for i a multiple of W).
 template <class T, class Src> struct Geo {
   Src src;
   int64_t lo, mid, hi;
@@ -428,9 +436,12 @@ __global__ __launch_bounds__(kBlock) void fold_tile_kernel(Geo<T, Src> g, int lg
   for (int k = 0; k < W; ++k) a[k] = F::ident();
   int64_t m = m0 + r;
   for (; m + 3 * R < m1; m += 4 * R) {
+    typename Src::Raw w[4];  // every load in flight before the first value
+#pragma unroll
+    for (int u = 0; u < 4; ++u) w[u] = g.src.raw(base + (m + u * R) * g.lo);
     V v[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) v[u] = g.src.vec(base + (m + u * R) * g.lo);
+    for (int u = 0; u < 4; ++u) v[u] = g.src.value(w[u]);
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
 #pragma unroll

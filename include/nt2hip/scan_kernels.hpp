@@ -57,8 +57,8 @@ namespace nt2h {
 
 // Scan geometry: element m of run o (the table's output o = i + lo*h) sits
 // at i + lo*(m + mid*h).  Elements are read from Src (a table, or an
-// expression evaluated per element: at(i), vec(i) for i a multiple of the
-// 16-byte width W) and written to the table y.
+// expression evaluated per element: at(i), and raw(i)/value()/vec(i) for i
+// a multiple of the 16-byte width W) and written to the table y.
 template <class T, class Src> struct SGeo {
   Src src;
   T* y;
@@ -381,9 +381,12 @@ __global__ __launch_bounds__(kBlock) void scan_colv_kernel(SGeo<T, Src> g, int64
   }
   int64_t m = m0;
   for (; m + 4 <= m1; m += 4) {
+    typename Src::Raw w[4];  // every load in flight before the first value
+#pragma unroll
+    for (int u = 0; u < 4; ++u) w[u] = g.src.raw(off + (m + u) * g.lo);
     VT v[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) v[u] = g.src.vec(off + (m + u) * g.lo);
+    for (int u = 0; u < 4; ++u) v[u] = g.src.value(w[u]);
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       VT r;
@@ -471,12 +474,15 @@ __global__ __launch_bounds__(kBlock) void scan_tile_kernel(SGeo<T, Src> g, int l
 #pragma unroll
     for (int k = 0; k < W; ++k) acc[k] = M::ident();
     for (int64_t mb = m0; mb < m1; mb += (int64_t)K * R) {
-      VT v[K];
+      typename Src::Raw w[K];  // every load in flight before the first value
 #pragma unroll
       for (int j = 0; j < K; ++j) {
         const int64_t m = mb + r + (int64_t)R * j;
-        v[j] = (live && m < m1) ? g.src.vec(off + m * g.lo) : VT{};
+        w[j] = (live && m < m1) ? g.src.raw(off + m * g.lo) : typename Src::Raw{};
       }
+      VT v[K];  // rows past the segment are masked where they are combined
+#pragma unroll
+      for (int j = 0; j < K; ++j) v[j] = g.src.value(w[j]);
 #pragma unroll
       for (int k = 0; k < W; ++k) {
         V a = M::ident();
@@ -507,12 +513,15 @@ __global__ __launch_bounds__(kBlock) void scan_tile_kernel(SGeo<T, Src> g, int l
   for (int k = 0; k < W; ++k) c[k] = (PASS == 3 && live) ? M::unpack(summ[(o + k) * so + s * ss]) : M::cident();
   for (int64_t mb = m0; mb < m1; mb += (int64_t)K * R) {
     const int64_t mr = mb + (int64_t)K * r;  // this lane's first row after the transposition
-    VT v[K];
+    typename Src::Raw w[K];  // every load in flight before the first value
 #pragma unroll
     for (int j = 0; j < K; ++j) {
       const int64_t m = mb + r + (int64_t)R * j;
-      v[j] = (live && m < m1) ? g.src.vec(off + m * g.lo) : VT{};
+      w[j] = (live && m < m1) ? g.src.raw(off + m * g.lo) : typename Src::Raw{};
     }
+    VT v[K];  // rows past the segment are masked where they are scanned
+#pragma unroll
+    for (int j = 0; j < K; ++j) v[j] = g.src.value(w[j]);
 #pragma unroll
     for (int j = 0; j < K; ++j) {
       const int t = gb + r + R * j;
@@ -759,11 +768,17 @@ __global__ __launch_bounds__(kBlock) void scan_lookback_kernel(SGeo<T, Src> g, i
   const int64_t run = o * g.mid;
   T* out = g.y + o * g.mid;
   // the whole tile in registers (mid % W == 0: a vector is all in or all out)
+  typename Src::Raw w[kLbSub];  // every load in flight before the first value
+#pragma unroll
+  for (int u = 0; u < kLbSub; ++u) {
+    const int64_t m = m0 + u * kSub + W * (int64_t)threadIdx.x;
+    w[u] = (m < m1) ? g.src.raw(run + m) : typename Src::Raw{};
+  }
   VT e[kLbSub];
 #pragma unroll
   for (int u = 0; u < kLbSub; ++u) {
     const int64_t m = m0 + u * kSub + W * (int64_t)threadIdx.x;
-    e[u] = (m < m1) ? g.src.vec(run + m) : VT{};
+    e[u] = (m < m1) ? g.src.value(w[u]) : VT{};
   }
   V p[kLbSub][W], inc[kLbSub];
 #pragma unroll

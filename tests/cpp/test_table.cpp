@@ -1,53 +1,23 @@
 // C++ front-end check: NT2-style table expressions on the GPU through
 // include/nt2hip/table.hpp, compared with the oracle (libnt2ref.so) on the
 // CPU.  Prints one "PASS name" / "FAIL name ..." line per check and exits
-// non-zero on any failure.  Built by tests/cpp/Makefile, run by
-// tests/test_gpu_frontend.py.
-#include <cmath>
-#include <cstdio>
-#include <cstring>
-#include <vector>
+// non-zero on any failure.  Built by tests/cpp/Makefile (with the nd_*.cpp
+// units), run by tests/test_gpu_frontend.py.
+#include "common.hpp"
 
-#include "../../include/nt2hip/table.hpp"
-
-extern "C" {
-void nt2ref_map_f32(int op, const float* x, float* y, int64_t n);
-void nt2ref_map_f64(int op, const double* x, double* y, int64_t n);
-void nt2ref_saxpy_f32(float a, const float* x, float* y, int64_t n);
-void nt2ref_bs_fwd_f32(const float*, const float*, const float*, const float*, const float*, float*, float*, int64_t);
-void nt2ref_exact_dot_f32(const float* x, const float* y, int64_t n, double* out);
-void nt2ref_exact_sum_f32(const float* x, int64_t n, double* out);
-void nt2ref_fold_lmh_f32(int op, const float* x, int64_t lo, int64_t mid, int64_t hi, float* out);
-void nt2ref_scan_lmh_f32(int op, const float* x, int64_t lo, int64_t mid, int64_t hi, float* y);
-}
-
-using nt2hip::table;
 namespace nt2 = nt2hip;  // NT2's demo bodies below compile as written
 static int failures = 0;
-static void report(const char* name, bool ok, const char* detail = "") {
+void report(const char* name, bool ok, const char* detail) {
   std::printf("%s %s %s\n", ok ? "PASS" : "FAIL", name, detail);
   if (!ok) ++failures;
 }
-static uint32_t bits(float f) { uint32_t u; std::memcpy(&u, &f, 4); return u; }
-static bool same_bits(const std::vector<float>& a, const std::vector<float>& b) {
-  if (a.size() != b.size()) return false;
-  for (size_t i = 0; i < a.size(); ++i)
-    if (bits(a[i]) != bits(b[i]) && !(std::isnan(a[i]) && std::isnan(b[i]))) return false;
-  return true;
-}
-static std::vector<float> uniform(size_t n, float lo, float hi, uint64_t seed) {
-  std::vector<float> v(n);
-  uint64_t z = seed;
-  for (auto& x : v) {
-    z += 0x9E3779B97F4A7C15ull;
-    uint64_t t = z;
-    t = (t ^ (t >> 30)) * 0xBF58476D1CE4E5B9ull;
-    t = (t ^ (t >> 27)) * 0x94D049BB133111EBull;
-    t ^= t >> 31;
-    x = (float)(lo + (hi - lo) * ((t >> 11) * 0x1.0p-53));
-  }
-  return v;
-}
+// the N-D checks (nd_*.cpp)
+void nd_checks_3d();
+void nd_checks_4d();
+void nd_checks_paths();
+void nd_checks_f64();
+void nd_checks_into_forms();
+void fused_fold_speed_check();
 
 // ---------------------------------------------------------------------------
 // NT2's own demo expressions, bodies copied unchanged (nt2:: is nt2hip::).
@@ -256,6 +226,12 @@ static void demo_checks() {
 
 int main() {
   demo_checks();
+  nd_checks_3d();
+  nd_checks_4d();
+  nd_checks_paths();
+  nd_checks_f64();
+  nd_checks_into_forms();
+  fused_fold_speed_check();
   const int64_t n = (1 << 20) + 7;
   auto hx = uniform(n, -1.f, 1.f, 1), hy = uniform(n, -1.f, 1.f, 2);
   table<float> X(hx), Y(hy);
