@@ -84,12 +84,12 @@ typedef enum nt2h_op {
 
 /* Arithmetic of the kernels.
  *  EXACT  NT2's scalar functor (two-rounding fma/horner, IEEE division):
- *         bit-identical to NT2's scalar path for every functor and both
- *         types, except float sin/cos of |x| > 2^18 pi, where NT2 runs fdlibm's
- *         float Payne-Hanek and this library the correctly rounded reduction
- *         (at most 2 ULP from NT2's result there, at most 1 ULP from the
- *         correctly rounded sin/cos), and double sin/cos above 2^20 pi/2
- *         (at most 2 ULP from NT2's result).
+ *         bit-identical to NT2's scalar path for every functor in float, and
+ *         in double except sin/cos above 2^20 pi/2, where NT2 runs fdlibm's
+ *         double Payne-Hanek and this library the correctly rounded
+ *         reduction (at most 2 ULP from NT2's result).  Float sin/cos above
+ *         2^18 pi reproduce fdlibm's float Payne-Hanek (checked on every
+ *         float).
  *  FUSED  NT2's algorithms and coefficients with FMA and a refined hardware
  *         reciprocal (within a stated ULP bound of EXACT).
  *  FAST   column fold only (nt2h_colfold_gc_f32_prec): hardware exp2

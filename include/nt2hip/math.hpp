@@ -22,6 +22,9 @@
 #include <stdint.h>
 
 #define NT2H_FD __device__ __forceinline__
+// host and device: the integer parts of the trigonometric reduction, which the
+// exhaustive CPU checks run from this same source
+#define NT2H_HD __host__ __device__ __forceinline__
 
 // No contraction from here to the end of the including translation unit
 // (clang has no push/pop for this on the AMDGPU target).
@@ -465,7 +468,8 @@ template <class P> NT2H_FD double tanh_d(double a0) {
 // ========================================================== trigonometry
 // 2/pi in 32-bit words, preceded by two zero words so that a window may
 // start before the binary point.  These are the bits of 2/pi (0.A2F9836E...).
-__device__ static const uint32_t kTwoOverPi32[] = {
+// (constexpr: one copy in device constant memory, readable from host code.)
+constexpr uint32_t kTwoOverPi32[] = {
     0x00000000u, 0x00000000u, 0xA2F9836Eu, 0x4E441529u, 0xFC2757D1u, 0xF534DDC0u, 0xDB629599u, 0x3C439041u,
     0xFE5163ABu, 0xDEBBC561u, 0xB7246E3Au, 0x424DD2E0u, 0x06492EEAu, 0x09D1921Cu, 0xFE1DEB1Cu, 0xB129A73Eu,
     0xE88235F5u, 0x2EBB4484u, 0xE99C7026u, 0xB45F7E41u, 0x3991D639u, 0x835339F4u, 0x9C845F8Bu, 0xBDF9283Bu,
@@ -477,7 +481,7 @@ __device__ static const uint32_t kTwoOverPi32[] = {
 // 32 bits of 2/pi starting at bit position pos (pos 0 = first fraction bit;
 // negative positions read zeros).  pos + 64 stays inside the table for every
 // finite double argument (max pos = 1023 - 52 + 3 = 974 < 46*32 - 64).
-NT2H_FD uint32_t twoopi_window32(int pos) {
+NT2H_HD uint32_t twoopi_window32(int pos) {
   int p = pos + 64;
   int w = p >> 5, s = p & 31;
   uint64_t hi = ((uint64_t)kTwoOverPi32[w] << 32) | kTwoOverPi32[w + 1];
@@ -647,21 +651,18 @@ template <class P> NT2H_FD int reduce_d(double x, double* xr) {
   *xr = rh;
   return q;
 }
-// The float band (2^6 pi, 2^28] of the Exact reduction.  Up to 2^18 pi NT2
+// The float band (2^6 pi, 2^18 pi] of the Exact reduction, where NT2
 // converts the float to double and runs its double medium reduction (five
 // products, nine differences; fdlibm's double medium for the last float,
-// 0x1.921fb6p+19); above it NT2 calls fdlibm's float Payne-Hanek, for which
-// these kernels give the correctly rounded xr instead.  Two FMAs with pi/2 in
-// two double pieces (the first exact: k*P1 and x are multiples of 2^-52 and
-// their difference is below 2) give, after rounding to float, NT2's xr and
-// quadrant on every float up to 2^18 pi and the correctly rounded xr with
-// the nearest quadrant on every float above, up to 2^28 — checked
-// exhaustively (tests/cpp/trig_band_check.cpp; the first float that
-// differs lies near 2^29, where the rounding of x*2/pi picks the other
-// quadrant at |xr| = pi/4).  Outside the band the result is meaningless
-// (callers select).
-constexpr float kExactBandLoF = 0x1.921fb6p+7f;  // exclusive
-constexpr float kExactBandHiF = 0x1.0p+28f;      // inclusive
+// 0x1.921fb6p+19).  Two FMAs with pi/2 in two double pieces (the first
+// exact: k*P1 and x are multiples of 2^-52 and their difference is below 2)
+// give, after rounding to float, NT2's xr and quadrant on every float of the
+// band — and the correctly rounded xr with the nearest quadrant on every
+// float above, up to 2^28 (both checked exhaustively,
+// tests/cpp/trig_band_check.cpp).  Outside the band the result is
+// meaningless (callers select).
+constexpr float kExactBandLoF = 0x1.921fb6p+7f;   // exclusive
+constexpr float kExactBandHiF = 0x1.921fb6p+19f;  // inclusive: fdlibm's float reduction above
 NT2H_FD bool in_exact_band_f(float x) { return x > kExactBandLoF && x <= kExactBandHiF; }
 NT2H_FD int reduce_exact_f_band(float x, float* xr) {
   const double xd = (double)x;
@@ -673,8 +674,8 @@ NT2H_FD int reduce_exact_f_band(float x, float* xr) {
 }
 // Float argument reduction, Exact policy: NT2's scalar ladder for mode big_
 // (trig_reduction.hpp:94-247) — straight (<= pi/2), Cephes 3-part (<= 20 pi),
-// fdlibm-medium in float (<= 2^6 pi) — then the band above up to 2^28
-// (x <= 2^28 here).
+// fdlibm-medium in float (<= 2^6 pi) — then the band above, up to 2^18 pi
+// (x <= 2^18 pi here).
 NT2H_FD int reduce_exact_f_near(float x, float* xr) {
   if (!(x > 0x1.921fb6p-1f)) { *xr = x; return 0; }
   if (!(x > 0x1.921fb6p+0f)) {
@@ -709,17 +710,211 @@ NT2H_FD int reduce_exact_f_near(float x, float* xr) {
   }
   return reduce_exact_f_band(x, xr);
 }
-// Beyond 2^28: the integer Payne-Hanek (correctly rounded xr).
-constexpr float kExactFarF = kExactBandHiF;
-NT2H_FD int reduce_exact_f_far(float x, float* xr) {
-  uint32_t b = __float_as_uint(x);
-  double rh, rl;
-  int q = payne_hanek((uint64_t)((b & 0x7fffffu) | 0x800000u), (int)(b >> 23) - 150, &rh, &rl);
-  *xr = (float)rh;
-  return q;
+// fdlibm's float reduction, which NT2 runs above 2^18 pi
+// (trig_reduction.hpp:227-232 -> __ieee754_rem_pio2f -> __kernel_rem_pio2f
+// with prec 2, scalar/rem_pio2.hpp:83-233; NT2 keeps the head y[0]),
+// restated on integers held in registers.  The method is Payne-Hanek as
+// fdlibm organises it for float: the mantissa in three bytes t0 t1 t2, 2/pi
+// in bytes P[], column sums q[i] = t0 P[jv+i] + t1 P[jv+i-1] + t2 P[jv+i-2]
+// of weight 2^(q0-8i) for the columns i <= J, carried into bytes; the
+// integer part gives the quadrant, a fraction >= 1/2 is complemented (next
+// quadrant), and while the fraction's leading bytes vanish more columns are
+// taken (J grows from 9 to at most 13 over the floats).  The head is
+// fdlibm's float sum, from the last column to the first, of
+// fq[u] = F[u] 2^(q0-7-8u) with F[u] = sum_{k<=9} c_k B[u-k] (c = the bytes
+// of pi/2, B = the fraction's bytes, B[0] the column of the integer part):
+// every F[u] is an integer below 2^19, so fq[u] is exact and the sum rounds
+// once per column as fdlibm's does.
+//
+// Here the bytes sit big-endian in four words, byte i of the frame being
+// column i, so that four columns at once are one product: the column sums of
+// bytes 4a .. 4a+3 are t0 W(jv+4a) + t1 W(jv+4a-1) + t2 W(jv+4a-2), W(p) the
+// 32-bit word of 2/pi bytes P[p .. p+3] (columns beyond J masked off), and
+// the carries run over four 64-bit words instead of thirteen bytes; the
+// complement is a negation of the fraction field, and F[u] are 4-byte dot
+// products with constant words.  Bit-identical to a statement-by-statement
+// restatement of fdlibm on every float above 2^18 pi
+// (tests/cpp/fdlibm_check.cpp).  x finite and > 2^18 pi.
+constexpr uint32_t kPio2Bytes[10] = {0xC9u, 0x0Fu, 0xDAu, 0xA2u, 0x21u, 0x68u, 0xC2u, 0x34u, 0xC4u, 0xC6u};
+constexpr float kFdlibmLoF = 0x1.921fb6p+19f;  // exclusive: NT2 reduces in double up to here
+NT2H_HD uint32_t dot4_u8(uint32_t a, uint32_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_udot4(a, b, 0u, false);
+#else
+  return (a & 255u) * (b & 255u) + ((a >> 8) & 255u) * ((b >> 8) & 255u) + ((a >> 16) & 255u) * ((b >> 16) & 255u) +
+         (a >> 24) * (b >> 24);
+#endif
 }
+// big-endian word of the constants that meet bytes 4a .. 4a+3 of the frame
+// in F[u]
+constexpr uint32_t pio2_word(int u, int a) {
+  uint32_t wv = 0;
+  for (int j = 0; j < 4; ++j) {
+    const int k = u - 4 * a - j;
+    wv |= ((k >= 0 && k <= 9) ? kPio2Bytes[k] : 0u) << (24 - 8 * j);
+  }
+  return wv;
+}
+NT2H_HD int reduce_fdlibm_f(float x, float* xr) {
+  constexpr int JMAX = 13;
+  const uint32_t b = __builtin_bit_cast(uint32_t, x);
+  const int e0 = (int)(b >> 23) - 134;  // ilogb(x) - 7, >= 12 here
+  const uint32_t m = (b & 0x7fffffu) | 0x800000u;
+  const uint64_t t0 = m >> 16, t1 = (m >> 8) & 255u, t2 = m & 255u;
+  const int jv = (e0 - 3) >> 3;
+  const int q0 = e0 - 8 * (jv + 1);  // -5 .. 2
+  // bytes jv-2 .. jv+17 of 2/pi as big-endian words (byte -1 reads zero),
+  // then W(jv + 4a - j) for the four frame words and j = 0, 1, 2
+  uint32_t w[5];
+#pragma unroll
+  for (int i = 0; i < 5; ++i) w[i] = twoopi_window32(8 * (jv - 2) + 32 * i);
+  uint32_t W0[4], W1[4], W2[4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    W2[a] = w[a];                                   // bytes jv+4a-2 ..
+    W1[a] = (w[a] << 8) | (w[a + 1] >> 24);         // bytes jv+4a-1 ..
+    W0[a] = (w[a] << 16) | (w[a + 1] >> 16);        // bytes jv+4a ..
+  }
+  int J = 9, n = 0, ih = 0;
+  uint64_t hi, lo;  // the fraction's bytes B[1..7] (56 bits) and B[8..13] (top 48 bits)
+  uint32_t zf;      // the fraction held in the integer part's column, in units of 2^q0 (q0 < 0)
+  for (;;) {
+    // column sums of the four frame words, columns beyond J masked off
+    uint64_t S[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      uint32_t mask = 0xffffffffu;
+      if (a == 2) mask = J >= 11 ? 0xffffffffu : (J == 10 ? 0xffffff00u : 0xffff0000u);
+      if (a == 3) mask = J >= 13 ? 0xffff0000u : (J == 12 ? 0xff000000u : 0u);
+      S[a] = t0 * (W0[a] & mask) + t1 * (W1[a] & mask) + t2 * (W2[a] & mask);
+    }
+    uint64_t v3 = S[3];
+    uint64_t v2 = S[2] + (v3 >> 32);
+    uint64_t v1 = S[1] + (v2 >> 32);
+    uint64_t v0 = S[0] + (v1 >> 32);
+    const uint32_t top = (uint32_t)(v0 >> 24);  // column 0 with everything carried into it
+    hi = ((v0 & 0xffffffull) << 32) | (v1 & 0xffffffffull);
+    lo = ((v2 & 0xffffffffull) << 32) | (v3 & 0xffffffffull);
+    if (q0 >= 0) {
+      n = (int)((top << q0) & 7u);
+      zf = 0;
+    } else {
+      n = (int)((top >> -q0) & 7u);
+      zf = top & ((1u << -q0) - 1u);
+    }
+    ih = 0;
+    if (q0 > 0) {  // the field's top q0 bits belong to the quadrant
+      n += (int)(hi >> (56 - q0));
+      hi &= 0x00ffffffffffffffull >> q0;
+      ih = (int)(hi >> (55 - q0));
+    } else if (q0 == 0) {
+      ih = (int)(hi >> 55);
+    } else if (zf >= (1u << (-q0 - 1))) {
+      ih = 2;
+    }
+    if (ih > 0) {  // fraction >= 1/2: next quadrant, 1 - fraction (fdlibm's bytewise complement)
+      n += 1;
+      const uint32_t borrow = (hi | lo) != 0 ? 1u : 0u;
+      const uint64_t nlo = (uint64_t)0 - lo;
+      hi = (~hi + (lo == 0 ? 1u : 0u)) & (0x00ffffffffffffffull >> (q0 > 0 ? q0 : 0));
+      lo = nlo;
+      if (ih == 2) zf = (1u << -q0) - zf - borrow;
+    }
+    if (zf == 0 && (J == 9 || (hi >> (56 - 8 * (J - 9))) == 0)) {
+      // leading bytes vanish: take columns up to 9 past the first nonzero byte
+      int s;
+      if (hi != 0) s = (__builtin_clzll(hi) - 8) / 8 + 1;
+      else if (lo != 0) s = __builtin_clzll(lo) / 8 + 8;
+      else s = 0;
+      if (s == 0 || s > JMAX - 9) break;  // no float gets here (checked on every float)
+      J = 9 + s;
+      continue;
+    }
+    break;
+  }
+  // F[u] as 4-byte dot products of the frame words with constant words
+  const uint32_t Bw[4] = {(zf << 24) | (uint32_t)(hi >> 32), (uint32_t)hi, (uint32_t)(lo >> 32), (uint32_t)lo};
+  float sum = 0.0f;
+#pragma unroll
+  for (int u = JMAX; u >= 0; --u) {
+    uint32_t F = 0;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const uint32_t cw = pio2_word(u, a);
+      if (cw != 0u) F += dot4_u8(Bw[a], cw);
+    }
+    if (u > J) F = 0;
+    sum = __builtin_fmaf((float)F, __builtin_ldexpf(1.0f, -8 * u), sum);
+  }
+  const float y0 = __builtin_ldexpf(sum, q0 - 7);
+  *xr = ih ? -y0 : y0;
+  return n & 3;
+}
+
+// The same result for x in (2^18 pi, 2^28] from the double reduced argument
+// of the two-FMA band reduction, when it is decidable from there; returns
+// false otherwise (about 4% of these floats), and the caller runs
+// reduce_fdlibm_f.  fdlibm's head is a float sum, last column first, of
+// nonnegative exact terms fq[u]; with s* the fraction's leading column,
+// y0 = fq[s*] + (fq[s*+1] + s_2) rounded twice, s_2 = RN(fq[s*+2] + s_3) and
+// s_3 the sum of the remaining columns, which differs from their exact total
+// T_3 by at most 2^-24 (T_3 + 7 T_4) (one rounding at T_3, at most seven at
+// columns below T_4).  The two-FMA reduction gives |r| = x mod pi/2 to about
+// 2^-53 relative; the fraction g = |r| 2/pi gives the four leading bytes
+// (hence fq[s*] .. fq[s*+3] exactly) and T_2 = |r| - fq[s*] - fq[s*+1]
+// is exact in double.  Unless T_2 lies within that bound of a float
+// rounding midpoint (or the bytes sit within 2^-14 of a boundary),
+// s_2 = RN(T_2), and the last two sums are float additions.  Bit-identical to
+// reduce_fdlibm_f wherever it returns true (tests/cpp/fdlibm_check.cpp walks
+// every float).
+constexpr float kFdlibmFastHiF = 0x1.0p+28f;  // inclusive
+NT2H_HD bool reduce_fdlibm_f_fast(float x, float* xr, int* quadrant) {
+  const double xd = (double)x;
+  const double k = __builtin_rint(xd * 0x1.45f306dc9c883p-1);
+  double r = __builtin_fma(-k, 0x1.921fb54442d18p+0, xd);
+  r = __builtin_fma(-k, 0x1.1a62633145c07p-54, r);
+  const double a = __builtin_fabs(r);
+  const uint32_t b = __builtin_bit_cast(uint32_t, x);
+  const int e0 = (int)(b >> 23) - 134;
+  const int q0 = e0 - 8 * (((e0 - 3) >> 3) + 1);
+  // g = |r| 2/pi; G = g 2^-q0: column 0 is its integer part, column s its
+  // s-th fraction byte
+  const double g = a * 0x1.45f306dc9c883p-1;
+  const int eg = (int)((__builtin_bit_cast(uint64_t, g) >> 52) & 0x7ff) - 1023 - q0;
+  const int s = eg >= 0 ? 0 : (7 - eg) >> 3;  // leading column
+  const double t = __builtin_ldexp(g, 8 * (s + 3) - q0);  // the four leading bytes, < 2^32
+  const double ti = __builtin_floor(t);
+  const double tf = t - ti;
+  const uint32_t I = (uint32_t)ti;
+  const uint32_t B0 = I >> 24, B1 = (I >> 16) & 255u, B2 = (I >> 8) & 255u, B3 = I & 255u;
+  const uint32_t F0 = kPio2Bytes[0] * B0, F1 = kPio2Bytes[0] * B1 + kPio2Bytes[1] * B0;
+  const uint32_t F2 = kPio2Bytes[0] * B2 + kPio2Bytes[1] * B1 + kPio2Bytes[2] * B0;
+  const uint32_t F3 = kPio2Bytes[0] * B3 + kPio2Bytes[1] * B2 + kPio2Bytes[2] * B1 + kPio2Bytes[3] * B0;
+  // in units of 2^(q0-7-8s), where fq[s+j] = F_j 2^-8j
+  const double A = __builtin_ldexp(a, 7 - q0 + 8 * s);
+  const double T2 = __builtin_fma(-(double)F1, 0x1.0p-8, A - (double)F0);
+  const double T3 = __builtin_fma(-(double)F2, 0x1.0p-16, T2);
+  const double T4 = __builtin_fma(-(double)F3, 0x1.0p-24, T3);
+  // |s_3 - T_3|: one rounding of s_3 and at most seven of columns at most T_4
+  const double eps = __builtin_fma(__builtin_fma(7.0, T4, T3), 0x1.002p-24, A * 0x1.0p-50);
+  // distance of T2 from the float rounding midpoints of its binade
+  const int e2 = (int)((__builtin_bit_cast(uint64_t, T2) >> 52) & 0x7ff) - 1023;
+  const double u2 = __builtin_ldexp(T2, 23 - e2);  // T2 in float ulps of its binade
+  const double dist = __builtin_fabs((u2 - __builtin_floor(u2)) - 0.5);
+  const bool ok = B0 != 0u && tf > 0x1.0p-14 && tf < 1.0 - 0x1.0p-14 && dist > __builtin_ldexp(eps, 23 - e2);
+  const float s2 = (float)T2;
+  const float s1 = (float)F1 * 0x1.0p-8f + s2;
+  const float y0 = __builtin_ldexpf((float)F0 + s1, q0 - 7 - 8 * s);
+  *xr = r < 0.0 ? -y0 : y0;
+  *quadrant = ((int32_t)k) & 3;
+  return ok;
+}
+
 NT2H_FD int reduce_exact_f(float x, float* xr) {
-  return (x > kExactFarF) ? reduce_exact_f_far(x, xr) : reduce_exact_f_near(x, xr);
+  if (!(x > kFdlibmLoF)) return reduce_exact_f_near(x, xr);
+  int q;
+  if (x <= kFdlibmFastHiF && reduce_fdlibm_f_fast(x, xr, &q)) return q;
+  return reduce_fdlibm_f(x, xr);
 }
 
 // f_trig_evaluation.hpp:40-48

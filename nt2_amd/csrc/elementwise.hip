@@ -145,22 +145,43 @@ template <class P> struct OpExp10 {
   __device__ __forceinline__ void operator()(const float (&in)[1], float (&out)[1]) const { out[0] = exp10_f<P>(in[0]); }
 };
 
-// EXACT float sin / cos over 16-byte vectors with the Payne-Hanek
-// arguments (|x| > 2^28) compacted.  The integer Payne-Hanek costs more
-// than the rest of the evaluation together, and run in place it occupies a
-// whole wave for each of the lane's four elements whenever any lane needs
-// it.  Instead each wave reduces the band (2^6 pi, 2^28] on every lane
-// without branching, runs the ladder of NT2's reduction only for lanes
-// holding smaller arguments, gathers the far ones (ballot + prefix count) into an LDS
-// list, reduces that list with all 64 lanes, and reads the results back; the
-// polynomial evaluation then runs on every element.  The arithmetic per
-// element is unchanged, so the results are those of sin_f / cos_f.  The
-// lists are per wave, so the waves synchronise on their own (no workgroup
-// barrier: a wave with many far arguments does not hold up the others).
+// EXACT float sin / cos over 16-byte vectors with the arguments above
+// 2^18 pi compacted.  There NT2 runs fdlibm's float Payne-Hanek; run in place
+// it would occupy a whole wave for each of the lane's four elements whenever
+// any lane needs it.  Instead each wave reduces the band (2^6 pi, 2^18 pi]
+// on every lane without branching, runs the ladder of NT2's reduction only
+// for lanes holding smaller arguments, and gathers the far ones (ballot +
+// prefix count) into an LDS list that all 64 lanes reduce with
+// reduce_fdlibm_f_fast.  The arguments it leaves undecided (about 1.5% of
+// those up to 2^28) and those beyond 2^28 are queued per wave in LDS with
+// their element index; their vector is stored with a provisional
+// value, and whenever 64 are queued (and once at the end) the wave reduces
+// them with reduce_fdlibm_f on all 64 lanes and stores each result over the
+// provisional one, after waiting for its own earlier stores to complete.
+// The arithmetic per element is that of sin_f / cos_f, so the results are.
+// Lists and queues are per wave, so the waves synchronise on their own (no
+// workgroup barrier: a wave with many far arguments does not hold up the
+// others).  Each wave runs several iterations so that the queue mostly
+// fills before it is drained.
+constexpr unsigned kTrigExactGrid = 16384;
 __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+// Reduces and stores the first cnt (<= 64) queued arguments of wave w.
+template <bool COS>
+__device__ __forceinline__ void trig_exact_drain(const float (&dqx)[kWaves][128], const int64_t (&dqi)[kWaves][128],
+                                                 int w, int lane, int cnt, float* __restrict__ y) {
+  // the provisional values were stored by this wave's earlier vector stores:
+  // let them complete before the same addresses are written again
+  __builtin_amdgcn_s_waitcnt(0);
+  if (lane < cnt) {
+    const float a0 = dqx[w][lane];
+    float r;
+    const int qq = reduce_fdlibm_f(__builtin_fabsf(a0), &r);
+    y[dqi[w][lane]] = COS ? cos_from<Exact>(a0, qq, r) : sin_from<Exact>(a0, qq, r);
+  }
 }
 template <bool COS>
 __global__ __launch_bounds__(kBlock) void trig_exact_kernel(const float* __restrict__ x, float* __restrict__ y,
@@ -168,51 +189,89 @@ __global__ __launch_bounds__(kBlock) void trig_exact_kernel(const float* __restr
   __shared__ float phx[kWaves][4 * 64];
   __shared__ float phr[kWaves][4 * 64];
   __shared__ int phq[kWaves][4 * 64];
+  __shared__ float dqx[kWaves][128];    // queued arguments (signed)
+  __shared__ int64_t dqi[kWaves][128];  // and their element indices
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));  // lanes under this one
   const int64_t nv = n / 4;
   const int64_t stride = (int64_t)gridDim.x * kBlock;
+  int nq = 0;  // queued entries of this wave (wave-uniform)
   for (int64_t base = (int64_t)blockIdx.x * kBlock; base < nv; base += stride) {
     const int64_t i = base + threadIdx.x;
     const bool act = i < nv;
     f32x4 v = act ? __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(x) + i) : f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    float xr[4];
-    int q[4], slot[4];
-    bool far[4];
+    // gather the arguments above 2^18 pi
+    int slot[4];
     int total = 0;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const float ax = __builtin_fabsf(v[e]);
-      const bool bad = !(ax <= 0x1.fffffep+127f);
-      far[e] = !bad && ax > kExactFarF;
-      // the band (all but 0.02% of arguments uniform in +-1e6) on every
-      // lane without a branch; NT2's ladder only where a lane holds a
-      // smaller argument
-      q[e] = reduce_exact_f_band(ax, &xr[e]);
-      if (!far[e] && !in_exact_band_f(ax)) q[e] = reduce_exact_f_near(bad ? 0.0f : ax, &xr[e]);
-      const uint64_t m = __ballot(far[e]);
-      slot[e] = total + __popcll(m & below);
-      if (far[e]) phx[w][slot[e]] = ax;
+      const bool far = ax > kFdlibmLoF && ax <= 0x1.fffffep+127f;
+      const uint64_t m = __ballot(far);
+      slot[e] = far ? total + __popcll(m & below) : -1;
+      if (far) phx[w][slot[e]] = ax;
       total += __popcll(m);
     }
     wave_sync();
     for (int j = lane; j < total; j += 64) {
-      float r;
-      phq[w][j] = reduce_exact_f_far(phx[w][j], &r);
+      const float ax = phx[w][j];
+      float r = 0.0f;
+      int qq = -1;  // queued
+      if (ax <= kFdlibmFastHiF && !reduce_fdlibm_f_fast(ax, &r, &qq)) qq = -1;
+      phq[w][j] = qq;
       phr[w][j] = r;
     }
     wave_sync();
     f32x4 o;
+    bool later[4];
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-      if (far[e]) {
-        xr[e] = phr[w][slot[e]];
-        q[e] = phq[w][slot[e]];
+      const float ax = __builtin_fabsf(v[e]);
+      const bool bad = !(ax <= 0x1.fffffep+127f);
+      float xr;
+      // the band on every lane without a branch; NT2's ladder only where a
+      // lane holds a smaller argument
+      int q = reduce_exact_f_band(ax, &xr);
+      if (slot[e] >= 0) {
+        xr = phr[w][slot[e]];
+        q = phq[w][slot[e]];
+      } else if (!in_exact_band_f(ax)) {
+        q = reduce_exact_f_near(bad ? 0.0f : ax, &xr);
       }
-      o[e] = COS ? cos_from<Exact>(v[e], q[e], xr[e]) : sin_from<Exact>(v[e], q[e], xr[e]);
+      later[e] = q < 0;
+      if (later[e]) q = 0;  // provisional
+      o[e] = COS ? cos_from<Exact>(v[e], q, xr) : sin_from<Exact>(v[e], q, xr);
     }
     if (act) __builtin_nontemporal_store(o, reinterpret_cast<f32x4*>(y) + i);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const uint64_t m = __ballot(later[e]);
+      if (m == 0) continue;  // wave-uniform
+      if (later[e]) {
+        const int sl = nq + __popcll(m & below);
+        dqx[w][sl] = v[e];
+        dqi[w][sl] = 4 * i + e;
+      }
+      nq += __popcll(m);
+      if (nq >= 64) {  // drain 64, move the rest (at most 63) to the front
+        wave_sync();
+        trig_exact_drain<COS>(dqx, dqi, w, lane, 64, y);
+        const bool mv = lane < nq - 64;
+        const float mx = mv ? dqx[w][64 + lane] : 0.0f;
+        const int64_t mi = mv ? dqi[w][64 + lane] : 0;
+        wave_sync();
+        if (mv) {
+          dqx[w][lane] = mx;
+          dqi[w][lane] = mi;
+        }
+        nq -= 64;
+      }
+    }
     wave_sync();  // the lists are rewritten by the next iteration
+  }
+  if (nq > 0) {
+    wave_sync();
+    trig_exact_drain<COS>(dqx, dqi, w, lane, nq, y);
   }
   const int64_t t = 4 * nv + (int64_t)blockIdx.x * kBlock + threadIdx.x;  // tail [4 nv, n)
   if (t < n) y[t] = COS ? cos_f<Exact>(x[t]) : sin_f<Exact>(x[t]);
@@ -222,7 +281,7 @@ template <class T, class P>
 static int map_dispatch(int op, const T* x, T* y, int64_t n, nt2h_stream_t s) {
   if constexpr (sizeof(T) == 4 && !P::fused) {
     if ((op == NT2H_OP_SIN || op == NT2H_OP_COS) && n > 0 && aligned16(x) && aligned16(y)) {
-      const unsigned g = stream_grid(n, 4, 65536);
+      const unsigned g = stream_grid(n, 4, kTrigExactGrid);
       if (op == NT2H_OP_SIN)
         hipLaunchKernelGGL((trig_exact_kernel<false>), dim3(g), dim3(kBlock), 0, as_hip(s), x, y, n);
       else

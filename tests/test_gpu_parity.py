@@ -103,25 +103,16 @@ def test_map_f32_exact_bit_identical(dev, ref, fn):
     got = ops.map(fn, _gpu(x, dev), nt2_amd.EXACT).cpu().numpy()
     want = ref.map_f32(fn, x)
     eq = _bits_equal(got, want)
-    if fn in ("sin", "cos"):
-        # beyond 2^18 pi NT2 reduces with fdlibm's float Payne-Hanek, the GPU
-        # with an integer Payne-Hanek giving the correctly rounded argument:
-        # allowed to differ by 2 std-ULP there, bit-exact everywhere else
-        big = np.abs(x) > np.float32(823549.6875)
-        assert eq[~big].all(), x[~big][~eq[~big]][:8]
-        st = ref.cmp_f32(got[big], want[big])
-        assert st["max_std_ulp"] <= 2 and st["nan_mismatch"] == 0
-    else:
-        assert eq.all(), (x[~eq][:8], got[~eq][:8], want[~eq][:8])
+    assert eq.all(), (x[~eq][:8], got[~eq][:8], want[~eq][:8])
 
 
 @pytest.mark.parametrize("fn", ["sin", "cos"])
 @pytest.mark.parametrize("lo,hi", [(-1e6, 1e6), (8.0e5, 8.4e5), (-3e7, 3e7), (-60, 60), (2.6e8, 2.8e8),
                                    (-3e8, 3e8), (-1e10, 1e10)])
 def test_trig_exact_compacted_equals_scalar_path(dev, fn, lo, hi):
-    """EXACT float sin/cos over aligned tables reduces the band (2^6 pi, 2^28]
+    """EXACT float sin/cos over aligned tables reduces the band (2^6 pi, 2^18 pi]
     on every lane, runs NT2's ladder for smaller arguments and gathers the
-    Payne-Hanek arguments (beyond 2^28) of each wave into one list; the
+    fdlibm arguments (beyond 2^18 pi) of each wave into one list; the
     scalar path (an unaligned table) evaluates every element in place.  Same
     results: bit-identical, tails and specials included; mixes of band, small
     and far arguments per wave from none to all."""
@@ -539,15 +530,17 @@ def test_bs_fwd_adj_empty_portfolio_totals(dev, fn):
 
 @pytest.mark.parametrize("fn", ["sin", "cos"])
 def test_trig_exact_band_strided(dev, ref, fn):
-    """Every 257th float of the reduction band (2^6 pi, 2^28] and the floats at
-    its edges, both signs, through the aligned EXACT kernel (two-FMA band
-    reduction on every lane): bit-identical to the oracle's NT2 ladder up to
-    2^18 pi, within 2 ULP of it above (NT2 uses fdlibm's float Payne-Hanek
-    there, the kernels the correctly rounded reduced argument)."""
+    """Every 257th float of (2^6 pi, 2^28], every 4099th float above up to the
+    largest float, and the floats at the edges of the reduction's ranges,
+    both signs, through the aligned EXACT kernel (two-FMA band reduction on
+    every lane, fdlibm's float Payne-Hanek for the gathered arguments above
+    2^18 pi): bit-identical to the oracle's NT2 ladder everywhere."""
     lo = np.float32(float.fromhex("0x1.921fb6p+7")).view(np.uint32)  # 2^6 pi (float)
     mid = np.float32(float.fromhex("0x1.921fb6p+19")).view(np.uint32)  # 2^18 pi (float)
     hi = np.float32(2.0 ** 28).view(np.uint32)
+    top = np.uint32(0x7F7FFFFF)
     bits = np.concatenate([np.arange(lo - 2, hi + 3, 257, dtype=np.uint32),
+                           np.arange(hi, top, 4099, dtype=np.uint32), np.arange(top - 4, top + 1, dtype=np.uint32),
                            np.arange(lo - 2, lo + 3, dtype=np.uint32), np.arange(mid - 2, mid + 3, dtype=np.uint32),
                            np.arange(hi - 2, hi + 3, dtype=np.uint32)])
     x = bits.view(np.float32)
@@ -555,8 +548,5 @@ def test_trig_exact_band_strided(dev, ref, fn):
     x = np.concatenate([x, np.zeros((-x.size) % 4, np.float32)])  # whole 16-byte vectors
     got = ops.map(fn, _gpu(x, dev), nt2_amd.EXACT).cpu().numpy()
     want = ref.map_f32(fn, x)
-    near = np.abs(x) <= np.float32(float.fromhex("0x1.921fb6p+19"))
-    eq = _bits_equal(got[near], want[near])
-    assert eq.all(), (x[near][~eq][:8], got[near][~eq][:8], want[near][~eq][:8])
-    st = ref.cmp_f32(got[~near], want[~near])
-    assert st["max_std_ulp"] <= 2 and st["nan_mismatch"] == 0
+    eq = _bits_equal(got, want)
+    assert eq.all(), (x[~eq][:8], got[~eq][:8], want[~eq][:8])
