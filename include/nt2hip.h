@@ -84,12 +84,10 @@ typedef enum nt2h_op {
 
 /* Arithmetic of the kernels.
  *  EXACT  NT2's scalar functor (two-rounding fma/horner, IEEE division):
- *         bit-identical to NT2's scalar path for every functor in float, and
- *         in double except sin/cos above 2^20 pi/2, where NT2 runs fdlibm's
- *         double Payne-Hanek and this library the correctly rounded
- *         reduction (at most 2 ULP from NT2's result).  Float sin/cos above
- *         2^18 pi reproduce fdlibm's float Payne-Hanek (checked on every
- *         float).
+ *         bit-identical to NT2's scalar path for every functor, float and
+ *         double; sin/cos reproduce fdlibm's Payne-Hanek where NT2 runs it
+ *         (float above 2^18 pi: checked on every float; double above
+ *         2^20 pi/2: checked on sampled doubles of every binade).
  *  FUSED  NT2's algorithms and coefficients with FMA and a refined hardware
  *         reciprocal (within a stated ULP bound of EXACT).
  *  FAST   column fold only (nt2h_colfold_gc_f32_prec): hardware exp2

@@ -588,9 +588,133 @@ NT2H_FD int reduce_fused_f(float x, float* xr) {
   return q;
 }
 
+// fdlibm's double reduction, which NT2 runs above 2^20 pi/2 for doubles
+// (trig_reduction.hpp:230-233 -> scalar/rem_pio2.hpp:415-428 ->
+// __ieee754_rem_pio2 (484-622) -> __kernel_rem_pio2 with prec 2 (735-913);
+// NT2 keeps the head y[0]), restated on integers held in registers, as reduce_fdlibm_f does
+// for float but with fdlibm's 24-bit columns: the mantissa in three chunks
+// t0 t1 t2 (24, 24 and 5 bits), 2/pi in 24-bit chunks P[], column sums
+// q[i] = t0 P[jv+i] + t1 P[jv+i-1] + t2 P[jv+i-2] below 2^50 for the columns
+// i <= J (J from 4), carried into 24-bit chunks B[i]; the same quadrant,
+// complement and "more columns" steps; the integer part's column B[0] may
+// hold up to 27 fraction bits, which fdlibm splits into two chunks.  The
+// head is fdlibm's double sum, from the last column to the first, of
+// F[u] 2^(q0-23-24u) with F[u] = sum_{k<=4} c_k B[u-k] (c = the 24-bit chunks
+// of pi/2): every F[u] is an integer below 2^51, exact in double.  Columns
+// are counted from the leading one, so every array index is a constant.
+// Compared with a statement-by-statement restatement of fdlibm on sampled
+// doubles of every binade and on the doubles closest to multiples of pi/2
+// (tests/cpp/fdlibm_check.cpp).  x finite and > 2^20 pi/2.
+constexpr uint32_t kPio2Chunks24[5] = {0xC90FDAu, 0xA22168u, 0xC234C4u, 0xC6628Bu, 0x80DC1Cu};
+NT2H_HD uint64_t twoopi_chunk24(int m) { return m < 0 ? 0u : (uint64_t)(twoopi_window32(24 * m) >> 8); }
+// One column sum t0 a + t1 b + t2 c of 24-bit integers, formed in double as
+// fdlibm forms it (products below 2^48, the sum below 2^50: exact).  Formed
+// as 64-bit integer products instead, gfx950 code from this compiler
+// narrowed one to a 32-bit multiply-add whose operand had lost its 24-bit
+// mask (wrong column sums on the GPU, right ones on the host).
+NT2H_HD uint64_t column_d(uint64_t t0, uint64_t t1, uint64_t t2, uint32_t a, uint32_t b, uint32_t c) {
+  const double s = __builtin_fma((double)t0, (double)a, __builtin_fma((double)t1, (double)b, (double)t2 * (double)c));
+  return (uint64_t)s;
+}
+NT2H_HD int reduce_fdlibm_d(double x, double* xr) {
+  constexpr int JK = 4, JMAX = 9;
+  const uint64_t bx = __builtin_bit_cast(uint64_t, x);
+  const int e0 = (int)((bx >> 52) & 0x7ff) - 1046;  // ilogb(x) - 23
+  const uint64_t m = (bx & 0xfffffffffffffull) | 0x10000000000000ull;
+  const uint64_t t0 = m >> 29, t1 = (m >> 5) & 0xffffffu, t2 = (m & 31u) << 19;
+  const int jv = e0 > 3 ? (e0 - 3) / 24 : 0;
+  const int q0 = e0 - 24 * (jv + 1);  // -27 .. 2
+  // 2/pi chunks jv-2 .. jv+JMAX (P[k+2] = chunk jv+k); the column sums are
+  // formed where they are carried
+  uint32_t P[JMAX + 3];
+#pragma unroll
+  for (int k = 0; k < JMAX + 3; ++k) P[k] = (uint32_t)twoopi_chunk24(jv + k - 2);
+  int J = JK, n = 0, ih = 0;
+  uint32_t B[JMAX + 1];
+  uint64_t zf;  // the fraction held in the integer part's column, in units of 2^q0 (q0 < 0)
+  for (;;) {
+    uint64_t c = 0;
+#pragma unroll
+    for (int i = JMAX; i >= 1; --i) {
+      const uint64_t v = (i <= J) ? column_d(t0, t1, t2, P[i + 2], P[i + 1], P[i]) + c : 0u;
+      B[i] = (uint32_t)(v & 0xffffffu);
+      c = v >> 24;
+    }
+    const uint64_t top = column_d(t0, t1, t2, P[2], P[1], P[0]) + c;
+    if (q0 >= 0) {
+      n = (int)((top << q0) & 7u);
+      zf = 0;
+    } else {
+      n = (int)((top >> -q0) & 7u);
+      zf = top & ((1ull << -q0) - 1u);
+    }
+    ih = 0;
+    if (q0 > 0) {
+      const uint32_t t = B[1] >> (24 - q0);
+      n += (int)t;
+      B[1] -= t << (24 - q0);
+      ih = (int)(B[1] >> (23 - q0));
+    } else if (q0 == 0) {
+      ih = (int)(B[1] >> 23);
+    } else if (zf >= (1ull << (-q0 - 1))) {
+      ih = 2;
+    }
+    if (ih > 0) {  // fraction >= 1/2: next quadrant, 1 - fraction
+      n += 1;
+      uint32_t borrow = 0;
+#pragma unroll
+      for (int i = JMAX; i >= 1; --i) {
+        if (i <= J) {
+          const uint32_t j = B[i];
+          B[i] = borrow ? 0xffffffu - j : (j ? 0x1000000u - j : 0u);
+          borrow |= (j != 0u) ? 1u : 0u;
+        }
+      }
+      if (q0 > 0) B[1] &= 0xffffffu >> q0;
+      if (ih == 2) zf = (1ull << -q0) - zf - borrow;
+    }
+    if (zf == 0) {
+      uint32_t lead = 0;
+#pragma unroll
+      for (int i = 1; i <= JMAX - JK; ++i)
+        if (i <= J - JK) lead |= B[i];
+      if (lead == 0) {  // leading chunks vanish: take columns up to 4 past the first nonzero chunk
+        int s = 0;
+#pragma unroll
+        for (int i = JMAX; i >= 1; --i)
+          if (i <= J && B[i] != 0u) s = i;
+        if (s == 0 || s > JMAX - JK) break;  // not reached by the doubles checked
+        J = JK + s;
+        continue;
+      }
+    }
+    break;
+  }
+  // the integer part's column, split as fdlibm splits it above 24 bits
+  const uint32_t Bt = (uint32_t)(zf >> 24), B0 = (uint32_t)(zf & 0xffffffu);
+  double sum = 0.0;
+#pragma unroll
+  for (int u = JMAX; u >= -1; --u) {
+    uint64_t F = 0;
+#pragma unroll
+    for (int k = 0; k <= 4; ++k) {
+      const int d = u - k;
+      if (d < -1) break;
+      const uint32_t bd = d == -1 ? Bt : (d == 0 ? B0 : ((d <= J) ? B[d] : 0u));
+      F += (uint64_t)kPio2Chunks24[k] * bd;
+    }
+    if (u > J) F = 0;
+    sum = __builtin_fma((double)F, __builtin_ldexp(1.0, -24 * u), sum);
+  }
+  const double y0 = __builtin_ldexp(sum, q0 - 23);
+  *xr = ih ? -y0 : y0;
+  return n & 3;
+}
+
 // Double reduction: NT2's double ladder (straight, Cephes, medium up to
 // 2^18 pi) then fdlibm's double medium up to 2^20 pi/2 (scalar/rem_pio2.hpp:
-// 484-612, with its cancellation checks), then Payne-Hanek.
+// 484-612, with its cancellation checks), then fdlibm's Payne-Hanek (Exact)
+// or the integer Payne-Hanek (Fused: the correctly rounded argument).
 template <class P> NT2H_FD int reduce_d(double x, double* xr) {
   if (!(x > 0x1.921fb54442d18p-1)) { *xr = x; return 0; }
   if (!(x > 0x1.921fb54442d18p+0)) {
@@ -646,6 +770,7 @@ template <class P> NT2H_FD int reduce_d(double x, double* xr) {
     *xr = y0;
     return n & 3;
   }
+  if constexpr (!P::fused) return reduce_fdlibm_d(x, xr);  // NT2's result bit for bit
   double rh, rl;
   int q = payne_hanek((bx & 0xfffffffffffffull) | 0x10000000000000ull, (int)((bx >> 52) & 0x7ff) - 1075, &rh, &rl);
   *xr = rh;
@@ -768,33 +893,29 @@ NT2H_HD int reduce_fdlibm_f(float x, float* xr) {
   uint32_t w[5];
 #pragma unroll
   for (int i = 0; i < 5; ++i) w[i] = twoopi_window32(8 * (jv - 2) + 32 * i);
-  uint32_t W0[4], W1[4], W2[4];
-#pragma unroll
-  for (int a = 0; a < 4; ++a) {
-    W2[a] = w[a];                                   // bytes jv+4a-2 ..
-    W1[a] = (w[a] << 8) | (w[a + 1] >> 24);         // bytes jv+4a-1 ..
-    W0[a] = (w[a] << 16) | (w[a + 1] >> 16);        // bytes jv+4a ..
-  }
   int J = 9, n = 0, ih = 0;
   uint64_t hi, lo;  // the fraction's bytes B[1..7] (56 bits) and B[8..13] (top 48 bits)
   uint32_t zf;      // the fraction held in the integer part's column, in units of 2^q0 (q0 < 0)
   for (;;) {
-    // column sums of the four frame words, columns beyond J masked off
-    uint64_t S[4];
+    // column sums of the four frame words (last first, carried upwards),
+    // columns beyond J masked off; W(jv+4a-j) = bytes jv+4a-j .. of 2/pi
+    uint32_t wd[4];
+    uint64_t v = 0;
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
+    for (int a = 3; a >= 0; --a) {
       uint32_t mask = 0xffffffffu;
       if (a == 2) mask = J >= 11 ? 0xffffffffu : (J == 10 ? 0xffffff00u : 0xffff0000u);
       if (a == 3) mask = J >= 13 ? 0xffff0000u : (J == 12 ? 0xff000000u : 0u);
-      S[a] = t0 * (W0[a] & mask) + t1 * (W1[a] & mask) + t2 * (W2[a] & mask);
+      const uint32_t W2 = w[a] & mask;
+      const uint32_t W1 = ((w[a] << 8) | (w[a + 1] >> 24)) & mask;
+      const uint32_t W0 = ((w[a] << 16) | (w[a + 1] >> 16)) & mask;
+      v = t0 * W0 + t1 * W1 + t2 * W2 + (v >> 32);
+      wd[a] = (uint32_t)v;
     }
-    uint64_t v3 = S[3];
-    uint64_t v2 = S[2] + (v3 >> 32);
-    uint64_t v1 = S[1] + (v2 >> 32);
-    uint64_t v0 = S[0] + (v1 >> 32);
+    const uint64_t v0 = v;
     const uint32_t top = (uint32_t)(v0 >> 24);  // column 0 with everything carried into it
-    hi = ((v0 & 0xffffffull) << 32) | (v1 & 0xffffffffull);
-    lo = ((v2 & 0xffffffffull) << 32) | (v3 & 0xffffffffull);
+    hi = ((v0 & 0xffffffull) << 32) | wd[1];
+    lo = ((uint64_t)wd[2] << 32) | wd[3];
     if (q0 >= 0) {
       n = (int)((top << q0) & 7u);
       zf = 0;

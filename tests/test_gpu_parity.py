@@ -5,6 +5,8 @@ arithmetic (saxpy, EXACT maps, EXACT Black-Scholes), a stated std-ULP bound
 for FUSED arithmetic, and relative/scale-normalised error bounds against
 the FP64 oracle for the compensated folds and the adjoints.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -174,13 +176,49 @@ def test_map_f64_exact(dev, ref, fn):
     got = ops.map(fn, _gpu(x, dev), nt2_amd.EXACT).cpu().numpy()
     want = ref.map_f64(fn, x)
     eq = _bits_equal(got, want)
-    if fn in ("sin", "cos"):
-        big = np.abs(x) > 1647099.3291652855  # 2^20 pi/2: Payne-Hanek on both sides
-        assert eq[~big].all(), x[~big][~eq[~big]][:8]
-        st = ref.ulp_stats_f64(got[big], want[big])
-        assert st["max_ulp"] <= 2
-    else:
-        assert eq.all(), (x[~eq][:8], got[~eq][:8], want[~eq][:8])
+    assert eq.all(), (x[~eq][:8], got[~eq][:8], want[~eq][:8])
+
+
+@pytest.mark.parametrize("fn", ["sin", "cos"])
+def test_trig_exact_wide_columns(dev, ref, fn):
+    """The floats nearest to multiples of pi/2, for which fdlibm's float
+    reduction takes 12 or 13 columns (tests/golden/fdlibm_wide_columns_f32.npy:
+    all 128 with 13, every 8th of the 26,887 with 12), both signs, through
+    the aligned kernel (gathered and queued) and an unaligned table (in
+    place): bit-identical to the oracle."""
+    bits = np.load(os.path.join(os.path.dirname(__file__), "golden", "fdlibm_wide_columns_f32.npy"))
+    assert bits.size == 3489
+    x = bits.view(np.float32)
+    x = np.concatenate([x, -x])
+    x = np.concatenate([x, np.zeros((-x.size) % 4, np.float32)])
+    want = ref.map_f32(fn, x)
+    got = ops.map(fn, _gpu(x, dev), nt2_amd.EXACT).cpu().numpy()
+    eq = _bits_equal(got, want)
+    assert eq.all(), (x[~eq][:8], got[~eq][:8], want[~eq][:8])
+    padded = _gpu(np.concatenate([[0], x]).astype(np.float32), dev)[1:]
+    got2 = ops.map(fn, padded, nt2_amd.EXACT).cpu().numpy()
+    eq2 = _bits_equal(got2, want)
+    assert eq2.all(), (x[~eq2][:8], got2[~eq2][:8], want[~eq2][:8])
+
+
+@pytest.mark.parametrize("fn", ["sin", "cos"])
+def test_map_f64_exact_trig_every_binade(dev, ref, fn):
+    """EXACT double sin/cos above 2^20 pi/2, where NT2 runs fdlibm's double
+    Payne-Hanek: 64 random doubles of every binade up to 2^1023, the doubles
+    around the one closest to a multiple of pi/2 and the largest double, both
+    signs: bit-identical to the oracle."""
+    rng = np.random.default_rng(31)
+    e = np.repeat(np.arange(21, 1024, dtype=np.uint64), 64)
+    bits = ((e + np.uint64(1023)) << np.uint64(52)) | (rng.integers(0, 1 << 52, e.size, dtype=np.uint64))
+    worst = np.float64(np.ldexp(6381956970095103.0, 797)).view(np.uint64)
+    bits = np.concatenate([bits, worst + np.arange(-8, 9, dtype=np.int64).astype(np.uint64),
+                           np.array([0x7FEFFFFFFFFFFFFF], dtype=np.uint64)])
+    x = bits.view(np.float64)
+    x = np.concatenate([x, -x])
+    got = ops.map(fn, _gpu(x, dev), nt2_amd.EXACT).cpu().numpy()
+    want = ref.map_f64(fn, x)
+    eq = _bits_equal(got, want)
+    assert eq.all(), (x[~eq][:8], got[~eq][:8], want[~eq][:8])
 
 
 def test_map_in_place_and_empty(dev, ref):

@@ -8,7 +8,10 @@ tests/cpp/trig_band_check compares them on every float of (2^6 pi, 2^18 pi]
 (modules/core/trigonometric/include/nt2/trigonometric/functions/scalar/rem_pio2.hpp:83-233);
 tests/cpp/fdlibm_check compares the GPU's register-only restatement
 (include/nt2hip/math.hpp reduce_fdlibm_f, compiled for the host) with the
-oracle's statement-by-statement one on every finite float above 2^18 pi.
+oracle's statement-by-statement one on every finite float above 2^18 pi,
+and reduce_fdlibm_d (fdlibm's double Payne-Hanek, which NT2 runs above
+2^20 pi/2) on 76 million sampled doubles of every binade and near multiples
+of pi/2.
 The kernels themselves are compared with the oracle on the GPU
 (test_gpu_parity.py)."""
 import os
@@ -33,3 +36,4 @@ def test_fdlibm_reduction_on_every_float():
     out = subprocess.run([exe], capture_output=True, text=True, timeout=600)
     assert out.returncode == 0, out.stdout + out.stderr
     assert "909570084 floats" in out.stdout and "(stride 1): 0 differ" in out.stdout, out.stdout
+    assert "doubles above 2^20 pi/2 (sampled): 0 differ" in out.stdout, out.stdout
