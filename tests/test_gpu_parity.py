@@ -469,6 +469,56 @@ def test_colfold_small_vs_oracle(dev, ref, precision):
     assert np.max(np.abs(z2.cpu().numpy() - wz) / mag) < 1e-6
 
 
+def test_colfold_fast_general_reduction(dev):
+    """FAST column fold with k so large that k x / pi exceeds 2^20 (the
+    general reduction of the float argument): the aligned kernel leaves such
+    groups of four to a second walk after its main loop, the unaligned one
+    evaluates every element in place with the same arithmetic.  Columns mix
+    ordinary k, k where some groups need the general path and k where all
+    do; rows not a multiple of four.  The two agree to 1e-6 of each column's
+    sum of |terms| (they differ in summation order only)."""
+    import torch
+
+    rows = 4096 + 3
+    kv = np.array([0.5, 10.0, 3.0e6, 1.0e7, 2.5e8, 7.0, 1.0e5, 4.0e6], dtype=np.float32)
+    cols = kv.size
+    rng = np.random.default_rng(9)
+    X = rng.standard_normal((cols, rows)).astype(np.float32)  # column-major: one row of X per column
+    s = rng.uniform(0.5, 2.0, cols).astype(np.float32)
+    p = rng.uniform(0.0, 2 * np.pi, cols).astype(np.float32)
+    gz = rng.uniform(-1.0, 1.0, cols).astype(np.float32)
+    assert np.any(np.abs(kv[2] * X[2] / np.pi) > 2 ** 20) and np.any(np.abs(kv[2] * X[2] / np.pi) < 2 ** 20)
+    ld_a = rows + 1  # ld % 4 == 0: the vector kernel
+    Xa = np.zeros((cols, ld_a), np.float32)
+    Xa[:, :rows] = X
+    ld_u = rows + 2  # odd leading dimension: the element-wise kernel
+    Xu = np.zeros((cols, ld_u), np.float32)
+    Xu[:, :rows] = X
+    gs, gk, gp, ggz = _gpu(s, dev), _gpu(kv, dev), _gpu(p, dev), _gpu(gz, dev)
+
+    def run(Xh, ld):  # through the C ABI: rows and the leading dimension explicit
+        gX = _gpu(Xh, dev)
+        outs = [torch.empty(cols, dtype=torch.float32, device=dev) for _ in range(4)]
+        nt2_amd.call("nt2h_colfold_gc_f32_prec", nt2_amd.FAST, gX.data_ptr(), rows, cols, ld, gs.data_ptr(),
+                     gk.data_ptr(), gp.data_ptr(), ggz.data_ptr(), *[t.data_ptr() for t in outs],
+                     torch.cuda.current_stream().cuda_stream)
+        torch.cuda.synchronize()
+        return outs
+
+    assert _gpu(Xa, dev).data_ptr() % 16 == 0
+    ga, gu = run(Xa, ld_a), run(Xu, ld_u)
+    X64 = X.astype(np.float64)
+    e = np.exp(-(X64 ** 2) / (2 * s[:, None].astype(np.float64) ** 2))
+    g = np.abs(gz.astype(np.float64))
+    scales = [e.sum(axis=1), g * (e * X64 ** 2).sum(axis=1) / s.astype(np.float64) ** 3,
+              g * (e * np.abs(X64)).sum(axis=1), g * e.sum(axis=1)]
+    for name, a, u, sc in zip(("z", "dz_ds", "dz_dk", "dz_dp"), ga, gu, scales):
+        a, u = a.cpu().numpy(), u.cpu().numpy()
+        assert np.isfinite(a).all() and np.isfinite(u).all(), name
+        err = np.max(np.abs(a.astype(np.float64) - u) / sc)
+        assert err < 1e-6, (name, err)
+
+
 # Stated bound of the column fold against the FP64 fold at 16384 x 8192, for
 # each output as a fraction of the column's sum of |terms| of that output
 # (|gz| e, |gz| e |X|, |gz| e X^2 / s^3), about twice the largest measured
