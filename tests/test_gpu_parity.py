@@ -180,6 +180,23 @@ def test_map_f64_exact(dev, ref, fn):
 
 
 @pytest.mark.parametrize("fn", ["sin", "cos"])
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 7, 9])
+def test_trig_exact_small_sizes(dev, ref, fn, n):
+    """EXACT float sin/cos of a few elements (aligned): below four only the
+    kernel's scalar tail runs; above, one vector plus a tail.  Arguments from
+    every reduction range, fdlibm's above 2^18 pi (fast path, queued,
+    beyond 2^28) among them: bit-identical to the oracle."""
+    pool = np.array([0.3, -2.0, 150.0, -5.0e5, 9.0e5, -3.0e7, 2.7e8, -1.0e10, 3.4e38, float.fromhex("0x1.921fb6p+19"),
+                     float.fromhex("0x1.921fb8p+19"), -float.fromhex("0x1.6ac5b2p+28")], dtype=np.float32)
+    for shift in range(pool.size):
+        x = np.roll(pool, shift)[:n].copy()
+        got = ops.map(fn, _gpu(x, dev), nt2_amd.EXACT).cpu().numpy()
+        want = ref.map_f32(fn, x)
+        eq = _bits_equal(got, want)
+        assert eq.all(), (shift, x[~eq], got[~eq], want[~eq])
+
+
+@pytest.mark.parametrize("fn", ["sin", "cos"])
 def test_trig_exact_wide_columns(dev, ref, fn):
     """The floats nearest to multiples of pi/2, for which fdlibm's float
     reduction takes 12 or 13 columns (tests/golden/fdlibm_wide_columns_f32.npy:
