@@ -221,11 +221,11 @@ def test_trig_exact_wide_columns(dev, ref, fn):
 @pytest.mark.parametrize("fn", ["sin", "cos"])
 def test_map_f64_exact_trig_every_binade(dev, ref, fn):
     """EXACT double sin/cos above 2^20 pi/2, where NT2 runs fdlibm's double
-    Payne-Hanek: 64 random doubles of every binade up to 2^1023, the doubles
+    Payne-Hanek: 2048 random doubles of every binade up to 2^1023, the doubles
     around the one closest to a multiple of pi/2 and the largest double, both
     signs: bit-identical to the oracle."""
     rng = np.random.default_rng(31)
-    e = np.repeat(np.arange(21, 1024, dtype=np.uint64), 64)
+    e = np.repeat(np.arange(21, 1024, dtype=np.uint64), 2048)
     bits = ((e + np.uint64(1023)) << np.uint64(52)) | (rng.integers(0, 1 << 52, e.size, dtype=np.uint64))
     worst = np.float64(np.ldexp(6381956970095103.0, 797)).view(np.uint64)
     bits = np.concatenate([bits, worst + np.arange(-8, 9, dtype=np.int64).astype(np.uint64),
