@@ -197,6 +197,42 @@ def test_trig_exact_small_sizes(dev, ref, fn, n):
 
 
 @pytest.mark.parametrize("fn", ["sin", "cos"])
+def test_trig_exact_every_float_above_2p18pi(dev, ref, fn):
+    """Every finite float above 2^18 pi (909,570,084 of them; positive, and
+    negated for every 7th chunk) through the aligned EXACT kernel, in chunks
+    of 2^26, against the oracle: bit-identical.  These are the arguments for
+    which NT2 runs fdlibm's float Payne-Hanek (fast path, queue and full
+    reduction on the GPU)."""
+    import sys
+    import time
+
+    import torch
+
+    lo = int(np.float32(float.fromhex("0x1.921fb6p+19")).view(np.uint32)) + 1
+    hi = 0x7F800000  # exclusive: +inf
+    chunk = 1 << 26
+    g = _gpu(np.zeros(chunk, np.float32), dev)
+    out = _gpu(np.zeros(chunk, np.float32), dev)
+    t0 = time.time()
+    for c, b0 in enumerate(range(lo, hi, chunk)):
+        b1 = min(b0 + chunk, hi)
+        x = np.arange(b0, b1, dtype=np.uint32).view(np.float32)
+        if c % 7 == 3:
+            x = -x
+        m = x.size + ((-x.size) % 4)
+        xs = np.zeros(m, np.float32)
+        xs[: x.size] = x
+        g[:m].copy_(torch.from_numpy(xs))
+        ops.map(fn, g[:m], nt2_amd.EXACT, out=out[:m])
+        got = out[: x.size].cpu().numpy()
+        want = ref.map_f32(fn, x)
+        eq = _bits_equal(got, want)
+        assert eq.all(), (x[~eq][:8], got[~eq][:8], want[~eq][:8])
+        print(f"{fn}: floats {b0:#x}..{b1 - 1:#x} bit-identical ({time.time() - t0:.1f} s)", flush=True)
+        sys.stdout.flush()
+
+
+@pytest.mark.parametrize("fn", ["sin", "cos"])
 def test_trig_exact_wide_columns(dev, ref, fn):
     """The floats nearest to multiples of pi/2, for which fdlibm's float
     reduction takes 12 or 13 columns (tests/golden/fdlibm_wide_columns_f32.npy:
