@@ -38,8 +38,9 @@ SPECIALS = [0.0, -0.0, 1.0, -1.0, 0.5, np.inf, -np.inf, np.nan, 1e-45, -1e-45, 1
 FUSED_ULP = {"exp": 2, "exp2": 2, "exp10": 2, "log": 2, "sqrt": 0, "erf": 4, "erfc": 80, "normcdf": 260,
              "normpdf": 80, "fastnormcdf": 16, "tanh": 4, "sin": 4, "cos": 4}
 # FUSED arithmetic against the FP64 evaluation (NT2's double functors), in
-# float ulps at the FP64 value (floored at the smallest normal)
-FUSED_TRUTH_ULP = {"exp": 1.5, "exp2": 1.5, "exp10": 2.5, "log": 1.5, "sqrt": 0.5, "erf": 3, "erfc": 18,
+# float ulps at the FP64 value (floored at the smallest normal); exp and exp2
+# reach 1.53 and 1.56 over every float (tests/test_gpu_exhaustive.py)
+FUSED_TRUTH_ULP = {"exp": 1.6, "exp2": 1.6, "exp10": 2.5, "log": 1.5, "sqrt": 0.5, "erf": 3, "erfc": 18,
                    "normcdf": 18, "normpdf": 3, "tanh": 2.5, "sin": 2, "cos": 2}
 
 
