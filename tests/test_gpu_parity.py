@@ -425,6 +425,21 @@ def test_bs_fwd_f64_exact_bit_identical(dev, ref):
     assert _bits_equal(p.cpu().numpy(), wp).all()
 
 
+def test_bs_fwd_config2_full_size_exact(dev, ref):
+    """The C2 workload itself: the Black-Scholes forward over 2^26 options
+    of the stated distributions (seed 0), EXACT, in float and in double:
+    every call and put bit-identical to the oracle's NT2 scalar evaluation."""
+    d = synth.bs_inputs(1 << 26)
+    keys = ("S", "K", "T", "r", "sig")
+    for dt in (np.float32, np.float64):
+        h = [d[k].astype(dt) for k in keys]
+        c, p = ops.bs_fwd(*[_gpu(v, dev) for v in h], precision=nt2_amd.EXACT)
+        wc, wp = ref.bs_fwd(*h, dtype=dt)
+        for got, want in ((c, wc), (p, wp)):
+            eq = _bits_equal(got.cpu().numpy(), want)
+            assert eq.all(), (dt, int((~eq).sum()))
+
+
 def _scaled_err(got, want, S):
     return np.max(np.abs(got.astype(np.float64) - want) / np.maximum(np.abs(want), 1e-6 * S))
 
