@@ -339,6 +339,30 @@ def test_sum_ill_conditioned_kahan(dev, ref):
     assert ops.sum(ga, nt2_amd.SUM_KAHAN).item() == k
 
 
+def test_sum_dot_config3_full_size(dev, ref):
+    """C3's reductions at their size: sum and dot of 2^28 floats drawn
+    N(1, 1e-3) (numpy's generator, seed 7), compensated (the default mode):
+    within one float ulp of the exact sum and dot, where NT2's sequential
+    float order is millions of ulps off; deterministic."""
+    import torch
+
+    n = 1 << 28
+    rng = np.random.default_rng(7)
+    a = rng.normal(1.0, 1e-3, n).astype(np.float32)
+    b = rng.normal(1.0, 1e-3, n).astype(np.float32)
+    ga, gb = torch.from_numpy(a).to(dev), torch.from_numpy(b).to(dev)
+    hi, lo = ref.exact_sum_f32(a)
+    dh, dl = ref.exact_dot_f32(a, b)
+    s1 = ops.sum(ga, nt2_amd.SUM_KAHAN).item()
+    d1 = ops.dot(ga, gb, nt2_amd.SUM_KAHAN).item()
+    assert _ulp_f32_vs(float(np.float32(s1)), hi + lo) <= 1, (s1, hi + lo)
+    assert _ulp_f32_vs(float(np.float32(d1)), dh + dl) <= 1, (d1, dh + dl)
+    assert ops.sum(ga, nt2_amd.SUM_KAHAN).item() == s1 and ops.dot(ga, gb, nt2_amd.SUM_KAHAN).item() == d1
+    seq = float(np.cumsum(a[: 1 << 26], dtype=np.float32)[-1])  # NT2's left-to-right order, first 2^26
+    sh, sl = ref.exact_sum_f32(a[: 1 << 26])
+    assert _ulp_f32_vs(seq, sh + sl) > 1000
+
+
 @pytest.mark.parametrize("n,unaligned,host_out", [((1 << 22) + 3, False, True), ((1 << 21) + 1, True, False),
                                                   ((1 << 21) + 1, True, True)])
 def test_sum_dot_two_launch_paths(dev, ref, n, unaligned, host_out):
