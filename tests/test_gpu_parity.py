@@ -449,6 +449,50 @@ def test_bs_fwd_f64_exact_bit_identical(dev, ref):
     assert _bits_equal(p.cpu().numpy(), wp).all()
 
 
+def _bs_wide_inputs(n, dtype):
+    """Black-Scholes streams far outside the C2 ranges: S and K log-uniform
+    over 1e-30..1e30 (S/K overflowing and underflowing), T over 1e-10..1e4,
+    r in [-0.5, 0.5], sigma over 1e-4..10 (d1 and d2 from ~0 to beyond
+    every erfc branch, N(d) exactly 0 and 1), a quarter near the money, and
+    each of 0, -0, +-inf, NaN, the smallest subnormal, -1 and 3.4e38 in every
+    stream against ordinary values in the others."""
+    rng = np.random.default_rng(77)
+    S = 10.0 ** rng.uniform(-30, 30, n)
+    K = 10.0 ** rng.uniform(-30, 30, n)
+    T = 10.0 ** rng.uniform(-10, 4, n)
+    r = rng.uniform(-0.5, 0.5, n)
+    sig = 10.0 ** rng.uniform(-4, 1, n)
+    q = n // 4
+    S[:q] = rng.uniform(5, 30, q)
+    K[:q] = S[:q] * rng.uniform(0.5, 2.0, q)
+    T[:q] = rng.uniform(0.01, 10, q)
+    sig[:q] = rng.uniform(0.01, 1.0, q)
+    specials = [0.0, -0.0, np.inf, -np.inf, np.nan, 1.401298464324817e-45, -1.0, 3.4e38]
+    for k, a in enumerate((S, K, T, r, sig)):
+        for j, v in enumerate(specials):
+            a[q + 8 * k + j] = v
+    return [a.astype(dtype) for a in (S, K, T, r, sig)]
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_bs_fwd_exact_wide_and_special_inputs(dev, ref, dtype):
+    """EXACT Black-Scholes forward bit-identical to NT2's scalar evaluation
+    (the oracle evaluates normcdf(-d1) and normcdf(-d2) on their own, the
+    kernel shares each erfc core between N(d) and N(-d)) on inputs that reach
+    every branch of erfc, the saturated tails, overflow of S/K, zero and
+    infinite volatility or maturity and NaN propagation; 2^16 + 5 options,
+    so the vector body and the scalar tail both run."""
+    h = _bs_wide_inputs((1 << 16) + 5, dtype)
+    c, p = ops.bs_fwd(*[_gpu(v, dev) for v in h], precision=nt2_amd.EXACT)
+    wc, wp = ref.bs_fwd(*h, dtype=dtype)
+    for got, want in ((c, wc), (p, wp)):
+        got = got.cpu().numpy()
+        eq = _bits_equal(got, want)
+        assert eq.all(), (int((~eq).sum()), np.flatnonzero(~eq)[:5], got[~eq][:5], want[~eq][:5])
+    # the inputs reach both erfc branches and the saturated tails
+    assert 0.05 < np.mean((wc != 0) & np.isfinite(wc)) < 0.95
+
+
 def test_bs_fwd_config2_full_size_exact(dev, ref):
     """The C2 workload itself: the Black-Scholes forward over 2^26 options
     of the stated distributions (seed 0), EXACT, in float and in double:
