@@ -195,7 +195,10 @@ int nt2h_bs_fwd_f64(int precision, const double* S, const double* K, const doubl
  * order — by the last-arriving block (small grids), or by a second small
  * launch over the stored partials (grids of 4096 blocks or more, where the
  * per-block tickets would cost bandwidth).  The same bits on every call for
- * a given size.  *out receives the sum as a double. */
+ * a given size.  *out receives the sum as a double.  Data holding +-inf or
+ * NaN, or whose partial sums overflow the float lanes, give the plain sum's
+ * IEEE value (+-inf; NaN for a NaN or for +inf with -inf): the
+ * compensation of an infinite running sum is dropped, not added. */
 int nt2h_sum_f32(const float* x, int64_t n, int mode, double* out, nt2h_stream_t s);
 int nt2h_dot_f32(const float* x, const float* y, int64_t n, int mode, double* out, nt2h_stream_t s);
 int nt2h_sum_f64(const double* x, int64_t n, double* out, nt2h_stream_t s);
@@ -217,7 +220,8 @@ int nt2h_dot_f64(const double* x, const double* y, int64_t n, double* out, nt2h_
  *    double), NaN and signed-zero behaviour included.
  *  - sum-like folds are compensated in the table's type with a compensated
  *    FP64 cross-segment combine (more accurate than NT2's sequential order,
- *    so not bit-identical);
+ *    so not bit-identical); +-inf and NaN come out where NT2's recurrence
+ *    has them (as for the whole-table sums above);
  *    prod keeps an FP64 mantissa with a separate exponent, so partial
  *    products never overflow (NT2's float product can, giving inf).
  *  - mid == 0 yields the neutral element (mean: NaN, as rec(0)*0). */
@@ -242,7 +246,7 @@ int nt2h_fold_f64(int op, const double* x, int64_t lo, int64_t mid, int64_t hi, 
  * cummin (min, Inf) — boost/simd/swar/functions/{cumsum,cumprod,cummax,
  * cummin}.hpp.  cummax/cummin are bit-identical to NT2's recurrence (NaN
  * resets included); cumsum uses compensated float pairs with FP64 carries
- * and cumprod an FP64 mantissa with a separate exponent (no intermediate
+ * (+-inf and NaN from where the recurrence reaches them) and cumprod an FP64 mantissa with a separate exponent (no intermediate
  * overflow).  y may equal x. */
 typedef enum nt2h_scan_op {
   NT2H_SCAN_CUMSUM = 0,

@@ -68,6 +68,13 @@ struct Comp {
   }
 };
 
+// The value s + c of a compensated pair.  An error term is non-finite only
+// when its sum overflowed or met an infinity or a NaN, and the running sum
+// stays non-finite from then on, so c is finite whenever s is; a
+// non-finite s is then the sum's own IEEE value (+-inf, or NaN for NaN data
+// or +inf + -inf), which adding c (NaN) would turn into NaN.
+template <class T> __device__ __forceinline__ T comp_value(T s, T c) { return __builtin_isfinite(s) ? s + c : s; }
+
 __device__ __forceinline__ float shfl_xor_f(float v, int m) { return __shfl_xor(v, m, 64); }
 __device__ __forceinline__ double shfl_xor_d(double v, int m) { return __shfl_xor(v, m, 64); }
 
@@ -99,7 +106,7 @@ __device__ __forceinline__ void block_reduce_comp(Comp (&acc)[NV], float* lds, d
     for (int v = 0; v < NV; ++v) {
       Comp a{lds[(v * kWaves) * 2], lds[(v * kWaves) * 2 + 1]};
       for (int w = 1; w < kWaves; ++w) a.merge(lds[(v * kWaves + w) * 2], lds[(v * kWaves + w) * 2 + 1]);
-      out[v] = (double)a.s + (double)a.c;
+      out[v] = comp_value((double)a.s, (double)a.c);
     }
   }
 }

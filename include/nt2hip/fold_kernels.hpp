@@ -156,6 +156,7 @@ template <int OP, class T> struct FSum {
   }
   // summaries: a compensated FP64 pair (s, c)
   __device__ static Summ summ(const L& a) {
+    if (!__builtin_isfinite(a.c.s)) return Summ{{(double)a.c.s, 0.0, 0.0, 0.0}};  // see comp_value
     double s, e;
     two_sum_t((double)a.c.s, (double)a.c.c, s, e);
     return Summ{{s, e, 0.0, 0.0}};
@@ -169,7 +170,7 @@ template <int OP, class T> struct FSum {
   }
   __device__ static Summ sident() { return Summ{{0.0, 0.0, 0.0, 0.0}}; }
   __device__ static T finish(const Summ& a, int64_t mid) {
-    T t = (T)(a.w[0] + a.w[1]);
+    T t = (T)comp_value(a.w[0], a.w[1]);
     if (OP == F_NORM2) return sqrt_t(t);
     if (OP == F_MEAN) {
       T f = T(1) / (T)mid;  // NT2: rec(value_type(size)) * sum

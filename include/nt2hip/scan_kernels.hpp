@@ -116,7 +116,7 @@ template <class T> struct MSum {
   __device__ static C cadd(const C& c, const V& v) { return ccomb(c, C{(double)v.s, (double)v.c}); }
   __device__ static T fin(const C& c, const V& v) {
     C r = cadd(c, v);
-    return (T)(r.s + r.c);
+    return (T)comp_value(r.s, r.c);
   }
   __device__ static SSumm pack(const C& c) { return SSumm{{c.s, c.c}}; }
   __device__ static C unpack(const SSumm& s) { return C{s.w[0], s.w[1]}; }

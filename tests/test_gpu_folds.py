@@ -411,3 +411,125 @@ def test_scan_in_place(dev, ref):
     nt2_amd.call("nt2h_scan_f32", nt2_amd.SCAN_OPS["cummax"], g.data_ptr(), 1, 70000, 1, g.data_ptr(),
                  torch.cuda.current_stream().cuda_stream)
     assert _bits_eq(g.cpu().numpy(), ref.scan_lmh("cummax", x, 1, 70000, 1)).all()
+
+
+# ------------------------------------------------- non-finite data in the sums
+# Every segment gets one of these; NT2's plain float recurrence gives +-inf
+# or NaN for each of them whatever the order, and the compensated kernels
+# must too (an error term that meets inf is NaN, which must not reach the
+# value).  "overflow": every value FLT_MAX/4 (DBL_MAX/4), so partial sums
+# overflow and the total's rounding to the element type is inf.
+NONFINITE = ("finite", "+inf", "-inf", "nan", "+inf-inf", "overflow")
+NONFINITE_SHAPES = [(1, 5000, 37), (7, 3000, 5), (16, 5000, 3), (1000, 3, 7), (4, 30000, 1), (12, 777, 5),
+                    (1, 100, 1000), (1024, 300, 2)]
+
+
+def _nonfinite_segments(lo, mid, hi, dtype, seed=31):
+    """(hi, mid, lo) data whose segment (h, i) is case NONFINITE[(h*lo+i) % 6]."""
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(0.5, 1.5, (hi, mid, lo)).astype(dtype)
+    big = np.finfo(dtype).max / 4
+    for h in range(hi):
+        for i in range(lo):
+            case = NONFINITE[(h * lo + i) % len(NONFINITE)]
+            k = rng.integers(0, mid, 2)
+            if case == "+inf":
+                x[h, k[0], i] = np.inf
+            elif case == "-inf":
+                x[h, k[0], i] = -np.inf
+            elif case == "nan":
+                x[h, k[0], i] = np.nan
+            elif case == "+inf-inf" and mid > 1:
+                x[h, k[0], i] = np.inf
+                x[h, (k[0] + 1 + k[1] % (mid - 1)) % mid, i] = -np.inf
+            elif case == "overflow":
+                x[h, :, i] = big
+    return x
+
+
+def _ieee_fold(op, x3, dtype):
+    """NT2's functor on (hi, mid, lo) data along mid, as the plain IEEE
+    recurrence gives it for these segments (order does not matter for them)."""
+    with np.errstate(all="ignore"):
+        w = x3.astype(np.float64)
+        if op == "asum1":
+            w = np.abs(w)
+        elif op in ("asum2", "norm2"):
+            w = w * w
+        s = w.sum(axis=1).astype(dtype)  # the total rounded to the element type: inf past its range
+        if op == "norm2":
+            s = np.sqrt(s)
+        elif op == "mean":
+            s = (dtype(1) / dtype(x3.shape[1])) * s
+    return s.reshape(-1)
+
+
+def _same_nonfinite(got, want, rtol):
+    got, want = np.asarray(got, np.float64), np.asarray(want, np.float64)
+    assert (np.isnan(got) == np.isnan(want)).all(), (np.flatnonzero(np.isnan(got) != np.isnan(want))[:5])
+    inf = np.isinf(want)
+    assert (got[inf] == want[inf]).all(), (got[inf][got[inf] != want[inf]][:5])
+    fin = np.isfinite(want)
+    assert np.isfinite(got[fin]).all()
+    assert np.allclose(got[fin], want[fin], rtol=rtol, atol=0), np.max(np.abs(got[fin] / want[fin] - 1))
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("shape", NONFINITE_SHAPES)
+def test_sum_family_and_cumsum_nonfinite(dev, dtype, shape):
+    """sum, mean, asum1, asum2, norm2 and cumsum of segments holding +inf,
+    -inf, NaN, both infinities or values whose sum overflows: +-inf and NaN
+    exactly where NT2's float recurrence has them, every other segment
+    finite and accurate (the column, tile and run kernels)."""
+    lo, mid, hi = shape
+    x3 = _nonfinite_segments(lo, mid, hi, dtype)
+    x = x3.reshape(-1)
+    rtol = 1e-6 if dtype == np.float32 else 1e-14
+    for op in ("sum", "mean", "asum1", "asum2", "norm2"):
+        _same_nonfinite(_fold(op, x, lo, mid, hi, dev), _ieee_fold(op, x3, dtype), rtol)
+    with np.errstate(all="ignore"):
+        want = np.cumsum(x3.astype(np.float64), axis=1).astype(dtype).reshape(-1)
+    # numpy's sequential FP64 cumsum is itself only within mid * eps
+    _same_nonfinite(_scan("cumsum", x, lo, mid, hi, dev), want, 1e-6 if dtype == np.float32 else 1e-9)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("n", [1000, (1 << 20) + 3, (1 << 24) + 1])
+def test_whole_table_sums_nonfinite(dev, dtype, n):
+    """Whole-table sum (all three summation modes), dot, the sum family and
+    cumsum with one +inf, -inf, NaN, both infinities, or an overflowing
+    total: the IEEE value of the plain sum, never a NaN from the
+    compensation (single- and two-launch grids, the vector body and the
+    scalar tail)."""
+    import torch
+
+    sfx = "f32" if dtype == np.float32 else "f64"
+    for case in NONFINITE[1:]:
+        x3 = np.random.default_rng(7).uniform(0.5, 1.5, (1, n, 1)).astype(dtype)
+        if case == "+inf":
+            x3[0, n // 3, 0] = np.inf
+        elif case == "-inf":
+            x3[0, n - 1, 0] = -np.inf  # in the scalar tail when n % 4 != 0
+        elif case == "nan":
+            x3[0, n // 2, 0] = np.nan
+        elif case == "+inf-inf":
+            x3[0, 1, 0], x3[0, n - 2, 0] = np.inf, -np.inf
+        else:
+            x3[:] = np.finfo(dtype).max / 4
+        x = x3.reshape(-1)
+        g = _dev_tensor(x, dev)
+        want = _ieee_fold("sum", x3, dtype)[0]
+        for mode in (nt2_amd.SUM_NAIVE, nt2_amd.SUM_KAHAN, nt2_amd.SUM_PAIRWISE):
+            if sfx == "f64" and mode != nt2_amd.SUM_KAHAN:
+                continue
+            _same_nonfinite([ops.sum(g, mode=mode).item()], [want], 1e-6)
+        with np.errstate(all="ignore"):
+            wd = np.dot(x.astype(np.float64), x.astype(np.float64))
+        _same_nonfinite([ops.dot(g, g).item()], [np.float64(dtype(wd)) if np.isfinite(wd) else wd], 1e-6)
+        for op in ("sum", "mean", "asum1", "asum2", "norm2"):
+            _same_nonfinite(ops.global_fold(op, g).cpu().numpy(), _ieee_fold(op, x3, dtype), 1e-6)
+        with np.errstate(all="ignore"):
+            wc = np.cumsum(x.astype(np.float64)).astype(dtype)
+        _same_nonfinite(_scan("cumsum", x, 1, n, 1, dev), wc, 1e-6 if dtype == np.float32 else 1e-9)
+        del g
+        torch.cuda.synchronize()
