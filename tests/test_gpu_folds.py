@@ -533,3 +533,66 @@ def test_whole_table_sums_nonfinite(dev, dtype, n):
         _same_nonfinite(_scan("cumsum", x, 1, n, 1, dev), wc, 1e-6 if dtype == np.float32 else 1e-9)
         del g
         torch.cuda.synchronize()
+
+
+PROD_CASES = ("finite", "+inf", "-inf", "nan", "0*inf", "-0", "overflow", "underflow", "underflow-neg")
+
+
+def _prod_segments(lo, mid, hi, dtype, seed=41):
+    """(hi, mid, lo) data whose segment (h, i) is case PROD_CASES[(h*lo+i) % 9]."""
+    rng = np.random.default_rng(seed)
+    # (finite segments stay far from the subnormals, where numpy's FP64
+    # recurrence would lose the bits the reference needs)
+    x = rng.uniform(0.99, 1.01, (hi, mid, lo)).astype(dtype)
+    big, small = (dtype(1e20), dtype(1e-20)) if dtype == np.float32 else (dtype(1e200), dtype(1e-200))
+    for h in range(hi):
+        for i in range(lo):
+            case = PROD_CASES[(h * lo + i) % len(PROD_CASES)]
+            k = rng.integers(0, mid, 2)
+            k1 = (k[0] + 1 + k[1] % (mid - 1)) % mid if mid > 1 else k[0]
+            if case == "+inf":
+                x[h, k[0], i] = np.inf
+            elif case == "-inf":
+                x[h, k[0], i] = -np.inf
+            elif case == "nan":
+                x[h, k[0], i] = np.nan
+            elif case == "0*inf":
+                x[h, k[0], i], x[h, k1, i] = 0.0, np.inf
+            elif case == "-0":
+                x[h, k[0], i] = -0.0
+            elif case == "overflow" and mid > 2:
+                x[h, :, i] = big
+            elif case == "underflow" and mid > 3:
+                x[h, :, i] = small
+            elif case == "underflow-neg" and mid > 3:
+                x[h, :, i] = -small
+    return x
+
+
+def _same_ieee(got, want, rtol):
+    """NaN, +-inf and signed zeros where want has them; the rest within rtol."""
+    got, want = np.asarray(got, np.float64), np.asarray(want, np.float64)
+    assert (np.isnan(got) == np.isnan(want)).all(), np.flatnonzero(np.isnan(got) != np.isnan(want))[:5]
+    spec = np.isinf(want) | (want == 0)
+    assert (got[spec] == want[spec]).all(), (got[spec][got[spec] != want[spec]][:5], want[spec][got[spec] != want[spec]][:5])
+    assert (np.signbit(got[spec]) == np.signbit(want[spec])).all()
+    fin = np.isfinite(want) & (want != 0)
+    assert np.allclose(got[fin], want[fin], rtol=rtol, atol=0), np.max(np.abs(got[fin] / want[fin] - 1))
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("shape", NONFINITE_SHAPES + [(1, (1 << 20) + 3, 1), (1, 1 << 18, 2)])
+def test_prod_and_cumprod_specials(dev, dtype, shape):
+    """prod and cumprod of segments holding +-inf, NaN, 0 with inf, -0, and
+    totals that overflow or underflow the element type (the FP64 mantissa
+    and separate exponent must end in +-inf and signed zeros exactly where
+    the true product does): the IEEE value of the product, rounded once."""
+    lo, mid, hi = shape
+    x3 = _prod_segments(lo, mid, hi, dtype)
+    x = x3.reshape(-1)
+    with np.errstate(all="ignore"):
+        wp = np.prod(x3.astype(np.float64), axis=1).astype(dtype).reshape(-1)
+        wc = np.cumprod(x3.astype(np.float64), axis=1).astype(dtype).reshape(-1)
+    rtol = 1e-6 if dtype == np.float32 else 1e-9
+    _same_ieee(_fold("prod", x, lo, mid, hi, dev), wp, rtol)
+    _same_ieee(_scan("cumprod", x, lo, mid, hi, dev), wc, rtol)
