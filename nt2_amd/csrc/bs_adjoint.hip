@@ -61,6 +61,12 @@ __device__ __forceinline__ BSAdjOut bs_fwd_adj_one(float S, float K, float Tm, f
   float n1 = 0.5f * ((t1 < 0.0f) ? 2.0f - r1 : r1);
   float t2 = -kInvSqrt2 * d2;
   float e2 = e1 * (sk * P::rcp(ert));
+  // exp(-d2^2/2) <= 1.  Beyond (inf, or NaN from 0 * inf) a factor of the
+  // identity left the float range: e^{-rT} underflowed to 0 (then kert is 0
+  // and N(d2) only has to be finite), or S/K or (S/K) e^{rT} overflowed (then
+  // K e^{-rT} < S 2^-128, and N(d2) enters the price below float
+  // resolution).  0 gives N(d2) in {0, 1} there instead of a NaN.
+  e2 = (e2 <= 2.0f) ? e2 : 0.0f;
   float r2 = erfc_pos_e<P>(__builtin_fabsf(t2), e2);
   float n2 = 0.5f * ((t2 < 0.0f) ? 2.0f - r2 : r2);
   float phi1 = e1 * kInvSqrt2Pi;
@@ -70,7 +76,11 @@ __device__ __forceinline__ BSAdjOut bs_fwd_adj_one(float S, float K, float Tm, f
   o.price = P::madd(S, n1, -kn2);
   o.dS = w * n1;
   o.dK = -(w * (ert * n2));
-  o.dT = w * P::madd(sphi * (0.5f * sig * sig), rsst, r * kn2);  // S phi sig / (2 sqrt T) = S phi sig^2 / (2 sig sqrt T)
+  // S phi sig / (2 sqrt T) = S phi sig^2 / (2 sig sqrt T), and 0 wherever
+  // S phi(d1) is (phi underflowed, S = 0), also when sig or sqrt T is 0
+  // (rsst = inf) or sig^2 overflows
+  const bool nophi = sphi == 0.0f;
+  o.dT = w * P::madd(nophi ? 0.0f : sphi * (0.5f * sig * sig), nophi ? 0.0f : rsst, r * kn2);
   o.dr = w * (Tm * kn2);
   o.dsig = w * (sphi * sqT);
   return o;

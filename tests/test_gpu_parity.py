@@ -493,6 +493,39 @@ def test_bs_fwd_exact_wide_and_special_inputs(dev, ref, dtype):
     assert 0.05 < np.mean((wc != 0) & np.isfinite(wc)) < 0.95
 
 
+def test_bs_wide_inputs_finite_where_nt2_is(dev, ref):
+    """FUSED forward and the fused forward+adjoint (float and double) on the
+    wide and special inputs above: the FUSED call and put are NaN or
+    infinite exactly where NT2's float path (EXACT) is, and every output of
+    the adjoint kernels is finite wherever NT2's float price and the FP64
+    oracle's value are (the adjoint's identities — exp(-d2^2/2) from
+    exp(-d1^2/2) (S/K) e^{rT}, sig/(2 sqrt T) as sig^2/(2 sig sqrt T) — must
+    not turn an underflowed discount factor, a zero volatility or maturity
+    or an overflowing sig^2 into NaN)."""
+    n = (1 << 16) + 5
+    h = _bs_wide_inputs(n, np.float32)
+    w = np.random.default_rng(3).uniform(-1, 1, n).astype(np.float32)
+    g = [_gpu(a, dev) for a in h + [w]]
+    ce, pe = [t.cpu().numpy() for t in ops.bs_fwd(*g[:5], precision=nt2_amd.EXACT)]
+    cf, pf = [t.cpu().numpy() for t in ops.bs_fwd(*g[:5], precision=nt2_amd.FUSED)]
+    for a, b in ((cf, ce), (pf, pe)):
+        assert (np.isnan(a) == np.isnan(b)).all() and (np.isinf(a) == np.isinf(b)).all()
+    want = ref.bs_fwd_adj(*[a.astype(np.float64) for a in h], w.astype(np.float64), dtype=np.float64)
+    adj32 = [o.cpu().numpy() for o in ops.bs_fwd_adj(*g, totals=False)[:6]]
+    adj64 = [o.cpu().numpy() for o in ops.bs_fwd_adj(*[_gpu(a.astype(np.float64), dev) for a in h + [w]],
+                                                     totals=False)[:6]]
+    for k, name in enumerate(("price", "dS", "dK", "dT", "dr", "dsig")):
+        m = np.isfinite(ce) & np.isfinite(want[k]) & (np.abs(want[k]) < 3e38)
+        assert m.sum() > n // 2
+        assert np.isfinite(adj32[k][m]).all(), (name, np.flatnonzero(m & ~np.isfinite(adj32[k]))[:5])
+        m64 = np.isfinite(want[k])
+        assert np.isfinite(adj64[k][m64]).all(), (name, np.flatnonzero(m64 & ~np.isfinite(adj64[k]))[:5])
+        nan_in = np.zeros(n, bool)
+        for a in h:
+            nan_in |= np.isnan(a)
+        assert np.isnan(adj32[k][nan_in]).all() and np.isnan(adj64[k][nan_in]).all(), name
+
+
 def test_bs_fwd_config2_full_size_exact(dev, ref):
     """The C2 workload itself: the Black-Scholes forward over 2^26 options
     of the stated distributions (seed 0), EXACT, in float and in double:
