@@ -1083,18 +1083,20 @@ template <class P> NT2H_FD float sin_f(float a0) {
   int n = reduce_f<P>(bad ? 0.0f : x, &xr);
   return sin_from<P>(a0, n, xr);
 }
-// sin and cos of one argument sharing the reduction (Fused kernels).
+// sin and cos of one argument sharing the reduction (Fused kernels); inf
+// and NaN give NaN, as sin_f and cos_f do.
 template <class P> NT2H_FD void sincos_f(float a0, float* s, float* c) {
   float x = __builtin_fabsf(a0), xr;
-  int n = reduce_f<P>(x, &xr);
+  bool bad = !(x <= 0x1.fffffep+127f);
+  int n = reduce_f<P>(bad ? 0.0f : x, &xr);
   float z = xr * xr;
   float se = sin_eval_f<P>(z, xr), ce = cos_eval_f<P>(z);
   bool sw = (n & 1) != 0;
   float sv = sw ? ce : se, cv = sw ? se : ce;
   uint32_t ssign = (__float_as_uint(a0) & 0x80000000u) ^ ((uint32_t)(n & 2) << 30);
   uint32_t csign = (uint32_t)(((n & 1) ^ ((n & 2) >> 1))) << 31;
-  *s = __uint_as_float(__float_as_uint(sv) ^ ssign);
-  *c = __uint_as_float(__float_as_uint(cv) ^ csign);
+  *s = bad ? __builtin_nanf("") : __uint_as_float(__float_as_uint(sv) ^ ssign);
+  *c = bad ? __builtin_nanf("") : __uint_as_float(__float_as_uint(cv) ^ csign);
 }
 
 template <class P> NT2H_FD double cos_eval_d(double z) {

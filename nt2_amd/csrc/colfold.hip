@@ -146,7 +146,8 @@ struct ColAcc {
       if (FAST) ex = __uint_as_float(__float_as_uint(ex) ^ m);
       float ec = ex * cs, es = ex * sn;
       t[0] = ec;
-      t[NV > 1 ? 1 : 0] = ec * x2;
+      // (ec x) x, not ec x^2: x^2 overflows for |x| > 2^64 while ex is 0
+      t[NV > 1 ? 1 : 0] = (ec * xv) * xv;
       t[NV > 2 ? 2 : 0] = es * xv;
       t[NV > 3 ? 3 : 0] = es;
     } else {
@@ -194,8 +195,10 @@ struct ColAcc {
       const float ex = __uint_as_float(__float_as_uint(__builtin_amdgcn_exp2f(x2 * cc.ce)) + (m[e] << 31));
       p[0] = __builtin_fmaf(ex, cv[e], p[0]);
       if (GRAD) {
-        p[NV > 1 ? 1 : 0] = __builtin_fmaf(ex * x2, cv[e], p[NV > 1 ? 1 : 0]);
-        p[NV > 2 ? 2 : 0] = __builtin_fmaf(ex * x[e], sv[e], p[NV > 2 ? 2 : 0]);
+        // ex x^2 as (ex x) x: x^2 overflows for |x| > 2^64 while ex is 0
+        const float exx = ex * x[e];
+        p[NV > 1 ? 1 : 0] = __builtin_fmaf(exx * x[e], cv[e], p[NV > 1 ? 1 : 0]);
+        p[NV > 2 ? 2 : 0] = __builtin_fmaf(exx, sv[e], p[NV > 2 ? 2 : 0]);
         p[NV > 3 ? 3 : 0] = __builtin_fmaf(ex, sv[e], p[NV > 3 ? 3 : 0]);
       }
     }

@@ -660,6 +660,38 @@ def test_colfold_small_vs_oracle(dev, ref, precision):
     assert np.max(np.abs(z2.cpu().numpy() - wz) / mag) < 1e-6
 
 
+@pytest.mark.parametrize("precision", [nt2_amd.FAST, nt2_amd.FUSED])
+def test_colfold_nonfinite_where_fp64_is(dev, ref, precision):
+    """Column fold with s over 1e-3..1e3, k up to 1e4 (the general reduction
+    beyond |k x / pi| = 2^20), p up to 1e4 and special values: columns with
+    +-inf, NaN or 1e30 in X (x^2 overflows while exp(-x^2/(2 s^2)) is 0), s
+    of 0, inf, 1e-30, 1e30 or NaN, k of 0, 1e30, inf or 3e7, p of 1e30 or
+    NaN.  z and the three gradients are NaN exactly where the FP64 oracle's
+    are and finite elsewhere (sin/cos of an infinite argument is NaN in
+    every path)."""
+    import torch
+
+    rows, cols = 1000, 512
+    rng = np.random.default_rng(9)
+    X = rng.standard_normal((rows, cols)).astype(np.float32)
+    s = (10.0 ** rng.uniform(-3, 3, cols)).astype(np.float32)
+    k = rng.uniform(0, 1e4, cols).astype(np.float32)
+    p = rng.uniform(-1e4, 1e4, cols).astype(np.float32)
+    X[3, 100], X[5, 101], X[7, 102], X[9, 103], X[:, 104] = np.inf, -np.inf, np.nan, 1e30, 0.0
+    s[105:110] = [0.0, np.inf, 1e-30, 1e30, np.nan]
+    k[110:113] = [0.0, 1e30, np.inf]
+    p[113], p[114], k[115] = 1e30, np.nan, 3e7
+    gz = rng.uniform(-1, 1, cols).astype(np.float32)
+    want = ref.colfold_gc_f64(X, s, k, p, gz)
+    Xg = torch.from_numpy(np.ascontiguousarray(X.T)).to(dev)
+    got = ops.colfold_gc(Xg, *[_gpu(a, dev) for a in (s, k, p, gz)], grad=True, precision=precision)
+    for name, a, b in zip(("z", "dz_ds", "dz_dk", "dz_dp"), got, want):
+        a = a.cpu().numpy().astype(np.float64)
+        assert (np.isnan(a) == np.isnan(b)).all(), (name, np.flatnonzero(np.isnan(a) != np.isnan(b)))
+        assert (np.isinf(a) == np.isinf(b)).all(), (name, np.flatnonzero(np.isinf(a) != np.isinf(b)))
+    assert np.isnan(want[0][[100, 101, 102]]).all() and np.isfinite(want[1][103])
+
+
 def test_colfold_fast_general_reduction(dev):
     """FAST column fold with k so large that k x / pi exceeds 2^20 (the
     general reduction of the float argument): the aligned kernel leaves such
