@@ -16,7 +16,11 @@ Metrics (each a maximum over the options):
                      S, S T, S sqrt(T))
   total_dr, total_dsig
                      |d sum g| / sum |g_i| (the w-signed terms cancel, so
-                     the error is stated against the sum of magnitudes)
+                     the error is stated against the sum of magnitudes);
+                     the terms' own float errors average out over a
+                     portfolio, so these two bounds are for portfolios of
+                     the measured sizes: over a few options a total is as
+                     accurate as its float terms (about 1e-7 of sum |g_i|)
 """
 from __future__ import annotations
 

@@ -562,6 +562,34 @@ def test_bs_fwd_adj_config4_full_size(dev, ref):
         assert abs(tot[k] - (hi + lo)) <= 1e-13 * np.sum(np.abs(got[j]).astype(np.float64)), k
 
 
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 7, 8, 9, 255, 257, 1027])
+def test_bs_small_portfolios(dev, ref, n):
+    """Portfolios of a few options (the scalar tail alone, one vector plus a
+    tail, one workgroup's totals): the EXACT forward bit-identical to the
+    oracle in float and double, the fused forward+adjoint within its stated
+    bounds, and its totals equal to the sum of the gradient streams it
+    wrote."""
+    d = synth.bs_inputs(n, seed=n, with_w=True)
+    keys = ("S", "K", "T", "r", "sig", "w")
+    for dt in (np.float32, np.float64):
+        h = [d[k].astype(dt) for k in keys[:5]]
+        c, p = ops.bs_fwd(*[_gpu(a, dev) for a in h], precision=nt2_amd.EXACT)
+        wc, wp = ref.bs_fwd(*h, dtype=dt)
+        assert _bits_equal(c.cpu().numpy(), wc).all() and _bits_equal(p.cpu().numpy(), wp).all(), dt
+    outs = ops.bs_fwd_adj(*[_gpu(d[k], dev) for k in keys])
+    got = [o.cpu().numpy() for o in outs[:6]]
+    tot = outs[6].cpu().numpy()
+    d64 = {k: d[k].astype(np.float64) for k in keys}
+    want = ref.bs_fwd_adj(*[d64[k] for k in keys], dtype=np.float64)
+    m = accuracy.metrics(tuple(d64[k] for k in keys), got, want[:6], tot, want[6])
+    # (the totals' bounds are for large portfolios, where the terms' float
+    # errors average out; here the totals are checked against the terms)
+    bad = [k for k in accuracy.check(m) if not k.startswith("total_")]
+    assert not bad, (bad, m)
+    for k, j in ((0, 4), (1, 5)):
+        assert abs(tot[k] - np.sum(got[j].astype(np.float64))) <= 1e-12 * np.sum(np.abs(got[j]).astype(np.float64))
+
+
 def _scaled_err(got, want, S):
     return np.max(np.abs(got.astype(np.float64) - want) / np.maximum(np.abs(want), 1e-6 * S))
 
