@@ -770,6 +770,46 @@ def test_colfold_fast_general_reduction(dev):
         assert err < 1e-6, (name, err)
 
 
+@pytest.mark.parametrize("precision", [nt2_amd.FAST, nt2_amd.FUSED])
+@pytest.mark.parametrize("rows,pad", [(1, 0), (2, 0), (3, 1), (4, 0), (5, 3), (8, 4), (255, 1), (1023, 5),
+                                      (1025, 3), (4099, 1), (4096, 0)])
+def test_colfold_odd_shapes(dev, ref, precision, rows, pad):
+    """Columns of 1 to 4099 rows with leading dimension rows + pad (the
+    vector kernel when it is a multiple of four, the element-wise one
+    otherwise; one partial round, only a tail, a lone element): z and the
+    three gradients against the FP64 oracle, to 4e-6 of each column's sum of
+    |terms|, and the forward-only launch's z equal to the full one's.  (A
+    column of a few rows carries the error of single terms: both policies
+    round the trigonometric argument k x + p, |.| <= 51 here, to float once,
+    which moves a term by up to 2^-24 x 51 = 3e-6 of its magnitude; over
+    thousands of rows these errors average out, hence the tighter bounds of
+    the full-size checks.)"""
+    import torch
+
+    cols, ld = 33, rows + pad
+    X, s, k, p = synth.colfold_inputs(rows, cols)  # X: (cols, rows), column-major
+    gz = synth.uniform(0, 78, cols, -1.0, 1.0)
+    Xp = np.zeros((cols, ld), np.float32)
+    Xp[:, :rows] = X
+    gX, gs, gk, gp, ggz = (_gpu(a, dev) for a in (Xp, s, k, p, gz))
+    outs = [torch.empty(cols, dtype=torch.float32, device=dev) for _ in range(4)]
+    nt2_amd.call("nt2h_colfold_gc_f32_prec", precision, gX.data_ptr(), rows, cols, ld, gs.data_ptr(), gk.data_ptr(),
+                 gp.data_ptr(), ggz.data_ptr(), *[t.data_ptr() for t in outs], torch.cuda.current_stream().cuda_stream)
+    z1 = torch.empty(cols, dtype=torch.float32, device=dev)
+    nt2_amd.call("nt2h_colfold_gc_f32_prec", precision, gX.data_ptr(), rows, cols, ld, gs.data_ptr(), gk.data_ptr(),
+                 gp.data_ptr(), None, z1.data_ptr(), None, None, None, torch.cuda.current_stream().cuda_stream)
+    want = ref.colfold_gc_f64(X.T, s, k, p, gz)
+    X64 = X.astype(np.float64)
+    e = np.exp(-(X64 ** 2) / (2 * s[:, None].astype(np.float64) ** 2))
+    g = np.abs(gz.astype(np.float64))
+    scales = [e.sum(axis=1), g * (e * X64 ** 2).sum(axis=1) / s.astype(np.float64) ** 3,
+              g * (e * np.abs(X64)).sum(axis=1), g * e.sum(axis=1)]
+    for name, a, w, sc in zip(("z", "dz_ds", "dz_dk", "dz_dp"), outs, want, scales):
+        err = np.max(np.abs(a.cpu().numpy().astype(np.float64) - w) / (sc + 1e-30))
+        assert err < 4e-6, (name, err)
+    assert np.array_equal(z1.cpu().numpy(), outs[0].cpu().numpy())
+
+
 # Stated bound of the column fold against the FP64 fold at 16384 x 8192, for
 # each output as a fraction of the column's sum of |terms| of that output
 # (|gz| e, |gz| e |X|, |gz| e X^2 / s^3), about 1.8 times the largest measured
