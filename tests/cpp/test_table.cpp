@@ -15,6 +15,7 @@ void report(const char* name, bool ok, const char* detail) {
 void nd_checks_3d();
 void nd_checks_4d();
 void nd_checks_paths();
+void nd_checks_edges();
 void nd_checks_f64();
 void nd_checks_into_forms();
 void fused_fold_speed_check();
@@ -229,6 +230,7 @@ int main() {
   nd_checks_3d();
   nd_checks_4d();
   nd_checks_paths();
+  nd_checks_edges();
   nd_checks_f64();
   nd_checks_into_forms();
   fused_fold_speed_check();
