@@ -94,6 +94,39 @@ def _cpu_baseline(inputs_np, budget_s=10.0):
     }
 
 
+def fd_residuals(g64, f32_out, outs64, m=1 << 16):
+    """Gradient check of the headline kernel on its first m options: central
+    differences (h = 1e-3 |x|, in FP64) of the FP64 call price for S, K, T, r
+    and sigma, times w, against the float kernel's greeks and the FP64
+    kernel's (the latter shows the differences' own truncation error); each
+    as max |g - w dP/dx| over the greek's natural scale (|w| x 1, e^{-rT}, S,
+    S T, S sqrt T), the scales of the stated bounds."""
+    import torch
+
+    import nt2_amd
+    from nt2_amd import ops
+
+    x = [t[:m].clone() for t in g64[:5]]
+    w = g64[5][:m]
+    S, K, T, r = x[0], x[1], x[2], x[3]
+    aw = w.abs()
+    scales = [aw, aw * torch.exp(-r * T), aw * S, aw * S * T, aw * S * torch.sqrt(T)]
+    out = {"options": m, "h": "1e-3 |x|, FP64 central differences", "f32_vs_fd": {}, "fp64_vs_fd": {}}
+    for j, nm in enumerate(("dS", "dK", "dT", "dr", "dsig")):
+        h = 1e-3 * x[j].abs()
+        up = [t.clone() for t in x]
+        dn = [t.clone() for t in x]
+        up[j] += h
+        dn[j] -= h
+        pu = ops.bs_fwd(*up, precision=nt2_amd.EXACT)[0]
+        pd = ops.bs_fwd(*dn, precision=nt2_amd.EXACT)[0]
+        g_fd = w * (pu - pd) / (up[j] - dn[j])
+        sc = torch.clamp(scales[j], min=1e-30)
+        out["f32_vs_fd"][nm] = float(((f32_out[j + 1][:m].double() - g_fd).abs() / sc).max())
+        out["fp64_vs_fd"][nm] = float(((outs64[j + 1][:m] - g_fd).abs() / sc).max())
+    return out
+
+
 def _events(stream, fn, reps):
     """Seconds per launch over `reps` back-to-back launches, after >= 30 ms of
     warm-up launches (clocks ramp up after the host-side input generation)."""
@@ -458,6 +491,17 @@ def report_colfold(dev, stream, rows=16384, cols=8192):
     zc, sn = e * torch.cos(arg), e * torch.sin(arg)
     ref = [zc.sum(1), (zc * Xd * Xd).sum(1) / sd[:, 0] ** 3, -(sn * Xd).sum(1), -sn.sum(1)]
     mag = [e.sum(1), (e * Xd * Xd).sum(1) / sd[:, 0] ** 3, (e * Xd.abs()).sum(1), e.sum(1)]
+    # central differences (h = 1e-3 |param|) of the FP64 fold for s, k, p
+    def zfold(sv, kv, pv):
+        return (torch.exp(-Xd * Xd / (2 * sv * sv)) * torch.cos(kv * Xd + pv)).sum(1)
+
+    fd = []
+    for j in range(3):
+        prm = [sd, kd, pd]
+        h = 1e-3 * prm[j].abs()
+        up, dn = list(prm), list(prm)
+        up[j], dn[j] = prm[j] + h, prm[j] - h
+        fd.append((zfold(*up) - zfold(*dn)) / (up[j] - dn[j])[:, 0])
     recs = []
     for name, prec in (("fast", nt2_amd.FAST), ("fused", nt2_amd.FUSED)):
         z, ds, dk, dp = ops.colfold_gc(X, s, k, p, precision=prec)
@@ -468,7 +512,15 @@ def report_colfold(dev, stream, rows=16384, cols=8192):
                      "fwd_adj_us": t * 1e6, "fwd_only_us": tf * 1e6,
                      "fwd_adj_gbps": 4 * rows * cols / t / 1e9, "hbm_frac": 4 * rows * cols / t / 8e12,
                      "fwd_only_hbm_frac": 4 * rows * cols / tf / 8e12, "elements_per_s": rows * cols / t,
-                     "err_vs_fp64_over_sum_abs_terms": dict(zip(("z", "dz_ds", "dz_dk", "dz_dp"), errs))})
+                     "err_vs_fp64_over_sum_abs_terms": dict(zip(("z", "dz_ds", "dz_dk", "dz_dp"), errs)),
+                     # gradient check: against FP64 central differences, and the FP64
+                     # analytic gradients against the same (the differences' truncation)
+                     "fd_residual_over_sum_abs_terms": {
+                         nm: float(((g[sel].double() - f).abs() / mm).max())
+                         for nm, g, f, mm in zip(("dz_ds", "dz_dk", "dz_dp"), (ds, dk, dp), fd, mag[1:])},
+                     "fp64_vs_fd_over_sum_abs_terms": {
+                         nm: float(((a - f).abs() / mm).max())
+                         for nm, a, f, mm in zip(("dz_ds", "dz_dk", "dz_dp"), ref[1:], fd, mag[1:])}})
     return recs
 
 
@@ -596,6 +648,7 @@ def main():
     m = accuracy.metrics(tuple(g64[:6]), f32_out, outs64, totals32, totals64)
     bound = {"measured": m, "bound": accuracy.BOUNDS, "violations": accuracy.check(m)}
     bound["pass"] = not bound["violations"]
+    fd = fd_residuals(g64, f32_out, outs64)
 
     # device copy of the same byte count (read 24 B + write 24 B per option)
     src = torch.empty(n * 6, dtype=torch.float32, device=dev)
@@ -677,6 +730,7 @@ def main():
         },
         "accuracy_vs_fp64": acc,
         "accuracy_bound": bound,
+        "gradient_check": fd,
     }
     if not args.no_cpu:
         line["cpu_baseline"] = _cpu_baseline(host)
