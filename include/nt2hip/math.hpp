@@ -159,8 +159,13 @@ template <class P> NT2H_FD double exp_d(double a0) {
   double c = P::nms(t, horner<P>(t, 0x1.6376972bea4d0p-25, -0x1.bbd41c5d26bf1p-20, 0x1.1566aaf25de2cp-14,
                                  -0x1.6c16c16bebd93p-9, 0x1.555555555553ep-3), x);
   c = 1.0 - ((lo - P::div(x * c, 2.0 - c)) - hi);
-  k = __builtin_fmin(__builtin_fmax(k, -1023.0), 1023.0);
-  double r = c * __longlong_as_double(((long long)k + 1023) << 52);
+  // below Maxlog k reaches 1024 (a0 in [1023.5 ln2, Maxlog)): NT2's scale
+  // (expo_scale.hpp:27-35) then builds the exponent field 2047, infinity,
+  // and returns inf where exp(a0) is still finite; the Fused policy scales
+  // by ldexp and returns the finite value.  (The clamp only keeps the
+  // integer conversion defined for arguments the selects below replace.)
+  k = __builtin_fmin(__builtin_fmax(k, -1023.0), 1024.0);
+  double r = P::fused ? __builtin_ldexp(c, (int)k) : c * __longlong_as_double(((long long)k + 1023) << 52);
   r = (a0 >= 0x1.62e42fefa39efp+9) ? kInfD : r;
   r = (a0 <= -0x1.6232bdd7abcd2p+9) ? 0.0 : r;
   return (a0 != a0) ? a0 : r;
@@ -196,8 +201,11 @@ template <class P> NT2H_FD float log_f(float a0) {
 // NT2 double log (d_log.hpp:52-112): fdlibm-style s = f/(2+f) kernel; no
 // subnormal prescale in the reference's double path (kept).
 template <class P> NT2H_FD double log_d(double a0) {
-  uint64_t b = ubits(a0);
-  int64_t k = (int64_t)((b & 0x7ff0000000000000ull) >> 52) - 1022;
+  // Fused policy: subnormals prescaled by 2^54 (NT2's double path reads
+  // their exponent field as if normal, which Exact keeps)
+  const bool den = P::fused && a0 > 0.0 && a0 < 0x1.0p-1022;
+  uint64_t b = ubits(den ? a0 * 0x1.0p+54 : a0);
+  int64_t k = (int64_t)((b & 0x7ff0000000000000ull) >> 52) - (den ? 1022 + 54 : 1022);
   double x = dbits((b & ~0x7ff0000000000000ull) | 0x3fe0000000000000ull);
   bool lt = 0x1.6a09e667f3bcdp-1 > x;
   k = lt ? k - 1 : k;
@@ -394,7 +402,10 @@ template <class P> NT2H_FD double erf_d(double a0) {
   if (!small && near) ex = exp_d<P>(-xx);
   double z = 1.0 - ex * q;
   z = a0 < 0.0 ? -z : z;
-  const double r = tiny ? q : small ? a0 * q : near ? z : 1.0;
+  // beyond 6 NT2 returns +1 whatever the sign (scalar/erf.hpp:69, reproduced
+  // under Exact); the Fused policy returns sign(a0)
+  const double far = P::fused ? __builtin_copysign(1.0, a0) : 1.0;
+  const double r = tiny ? q : small ? a0 * q : near ? z : far;
   return (a0 != a0) ? a0 : r;
 }
 

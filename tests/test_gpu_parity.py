@@ -275,6 +275,69 @@ def test_map_f64_exact_trig_every_binade(dev, ref, fn):
     assert eq.all(), (x[~eq][:8], got[~eq][:8], want[~eq][:8])
 
 
+@pytest.mark.parametrize("fn", F64_OPS)
+def test_map_f64_exact_every_binade(dev, ref, fn):
+    """EXACT double functors over 512 random doubles of every binade of the
+    whole double range (subnormals included), both signs, plus +-0, +-inf,
+    NaN, the largest double and the smallest subnormal: bit-identical to the
+    oracle's restatement of NT2's double path (every branch of every ladder
+    and interval is reached)."""
+    rng = np.random.default_rng(32)
+    e = np.repeat(np.arange(0, 2047, dtype=np.uint64), 512)  # biased exponents 0 (subnormals) .. 2046
+    bits = (e << np.uint64(52)) | rng.integers(0, 1 << 52, e.size, dtype=np.uint64)
+    special = np.array([0.0, np.inf, np.nan, np.finfo(np.float64).max, 5e-324, 1.0, 0.5], np.float64)
+    x = np.concatenate([bits.view(np.float64), special])
+    x = np.concatenate([x, -x])
+    got = ops.map(fn, _gpu(x, dev), nt2_amd.EXACT).cpu().numpy()
+    want = ref.map_f64(fn, x)
+    eq = _bits_equal(got, want)
+    assert eq.all(), (int((~eq).sum()), x[~eq][:8], got[~eq][:8], want[~eq][:8])
+
+
+# FUSED double functors against numpy's long double (64-bit significand)
+# functions, in double ulps at the true value, over every binade
+FUSED_F64_TRUTH_ULP = {"exp": 1.0, "log": 1.0, "sqrt": 0.5, "tanh": 1.5, "sin": 2.0, "cos": 2.0}
+
+
+@pytest.mark.parametrize("fn", F64_OPS)
+def test_map_f64_fused_every_binade(dev, fn):
+    """FUSED double functors over 512 random doubles of every binade, both
+    signs: exp, log, sqrt, tanh, sin and cos within FUSED_F64_TRUTH_ULP of
+    the long double value wherever it is a normal double (exp's finite
+    results below its overflow threshold included, where NT2's scale gives
+    inf; log's subnormal arguments included, which NT2's double path
+    misreads); erf, erfc, normcdf and normpdf within 8 ulps of the EXACT
+    (NT2) double result, and erf of x < -6 exactly -1 (NT2 returns +1)."""
+    rng = np.random.default_rng(33)
+    e = np.repeat(np.arange(0, 2047, dtype=np.uint64), 512)
+    bits = (e << np.uint64(52)) | rng.integers(0, 1 << 52, e.size, dtype=np.uint64)
+    x = bits.view(np.float64)
+    x = np.concatenate([x, -x, np.linspace(709.4, 709.78, 4001)])
+    g = _gpu(x, dev)
+    fu = ops.map(fn, g, nt2_amd.FUSED).cpu().numpy()
+    with np.errstate(all="ignore"):
+        if fn in FUSED_F64_TRUTH_ULP:
+            L = np.longdouble
+            t = {"exp": np.exp, "log": np.log, "sqrt": np.sqrt, "tanh": np.tanh, "sin": np.sin,
+                 "cos": np.cos}[fn](x.astype(L))
+            ok = np.isfinite(t) & (np.abs(t) >= L(2.2250738585072014e-308)) & (np.abs(t) <= L(1.7976931348623157e308))
+            assert np.isfinite(fu[ok]).all(), x[ok & ~np.isfinite(fu)][:5]
+            sp = np.spacing(np.abs(t[ok]).astype(np.float64)).astype(L)
+            u = (np.abs(fu[ok].astype(L) - t[ok]) / sp).astype(np.float64)
+            assert u.max() <= FUSED_F64_TRUTH_ULP[fn], (u.max(), x[ok][np.argmax(u)])
+            assert (np.isnan(fu) == np.isnan(t.astype(np.float64))).all()
+        else:
+            ex = ops.map(fn, g, nt2_amd.EXACT).cpu().numpy()
+            if fn == "erf":
+                neg = x < -6.0
+                assert (fu[neg] == -1.0).all() and (ex[neg] == 1.0).all()
+                ex = np.where(neg, -1.0, ex)
+            assert (np.isnan(fu) == np.isnan(ex)).all() and (np.isinf(fu) == np.isinf(ex)).all()
+            m = np.isfinite(ex) & (np.abs(ex) >= 2.2250738585072014e-308)
+            d = np.abs(fu[m] - ex[m]) / np.spacing(np.abs(ex[m]))
+            assert d.max() <= 8, (d.max(), x[m][np.argmax(d)])
+
+
 def test_map_in_place_and_empty(dev, ref):
     import torch
 
