@@ -85,11 +85,15 @@ typedef enum nt2h_op {
 /* Arithmetic of the kernels.
  *  EXACT  NT2's scalar functor (two-rounding fma/horner, IEEE division):
  *         bit-identical to NT2's scalar path for every functor, float and
- *         double; sin/cos reproduce fdlibm's Payne-Hanek where NT2 runs it
- *         (float above 2^18 pi: checked on every float; double above
- *         2^20 pi/2: checked on sampled doubles of every binade).
+ *         double, NT2's quirks included (double exp is inf on [709.436,
+ *         709.783), double log misreads subnormals, double erf is +1 below
+ *         -6); sin/cos reproduce fdlibm's Payne-Hanek where NT2 runs it
+ *         (float: every float checked; double: sampled doubles of every
+ *         binade).
  *  FUSED  NT2's algorithms and coefficients with FMA and a refined hardware
- *         reciprocal (within a stated ULP bound of EXACT).
+ *         reciprocal, within a stated ULP bound of EXACT, except where
+ *         NT2's double path is wrong: there FUSED gives the finite exp, the
+ *         prescaled subnormal log and erf = -1 below -6.
  *  FAST   column fold only (nt2h_colfold_gc_f32_prec): hardware exp2
  *         (v_exp_f32) and a reduction of the argument prescaled by 1/pi;
  *         each term's error is within a few 2^-24 of |term| plus 2^-24 of
