@@ -294,6 +294,39 @@ def test_map_f64_exact_every_binade(dev, ref, fn):
     assert eq.all(), (int((~eq).sum()), x[~eq][:8], got[~eq][:8], want[~eq][:8])
 
 
+# the branch points of NT2's double functors and reductions (math.hpp cites
+# each): exp's Maxlog, Minlog and 1023.5 ln2; erf/erfc intervals; tanh's
+# 0.625 and its saturation; sin/cos ladder bounds pi/4, pi/2, 20 pi,
+# 2^19 pi, fdlibm's medium limit (high word 0x413921fb) and 2^20 pi/2
+F64_BRANCH_POINTS = [float.fromhex(h) for h in (
+    "0x1.62e42fefa39efp+9", "-0x1.6232bdd7abcd2p+9", "0x1.921fb54442d18p-1", "0x1.921fb54442d18p+0",
+    "0x1.f6a7a2955385ep+5", "0x1.921fb54442d18p+19", "0x1.921fbp+20", "0x1.921fb54442d18p+20")] + [
+    1023.5 * 0.6931471805599453, 1e-10, 0.65, 2.2, 6.0, 2.0 / 3.0, 0.625, 22.0, 2.2250738585072014e-308,
+    0.7071067811865476, 1.4142135623730951, 1.0, 0.5]
+
+
+@pytest.mark.parametrize("fn", F64_OPS)
+def test_map_f64_exact_branch_points(dev, ref, fn):
+    """EXACT double functors at every branch point above and at the three
+    doubles on either side of it, both signs (and their halves, doubles and
+    square roots, which put erfc/normcdf/normpdf arguments on the same
+    points): bit-identical to the oracle."""
+    base = np.array(F64_BRANCH_POINTS, np.float64)
+    base = np.concatenate([base, base / np.sqrt(2.0), base * np.sqrt(2.0), base / 2, base * 2])
+    pts = [base]
+    for k in range(1, 4):
+        up, dn = base.copy(), base.copy()
+        for _ in range(k):
+            up, dn = np.nextafter(up, np.inf), np.nextafter(dn, -np.inf)
+        pts += [up, dn]
+    x = np.concatenate(pts)
+    x = np.concatenate([x, -x])
+    got = ops.map(fn, _gpu(x, dev), nt2_amd.EXACT).cpu().numpy()
+    want = ref.map_f64(fn, x)
+    eq = _bits_equal(got, want)
+    assert eq.all(), (x[~eq][:8], got[~eq][:8], want[~eq][:8])
+
+
 # FUSED double functors against numpy's long double (64-bit significand)
 # functions, in double ulps at the true value, over every binade
 FUSED_F64_TRUTH_ULP = {"exp": 1.0, "log": 1.0, "sqrt": 0.5, "tanh": 1.5, "sin": 2.0, "cos": 2.0}
