@@ -371,6 +371,23 @@ def test_map_f64_fused_every_binade(dev, fn):
             assert d.max() <= 8, (d.max(), x[m][np.argmax(d)])
 
 
+@pytest.mark.parametrize("precision", [nt2_amd.EXACT, nt2_amd.FUSED])
+def test_map_unaligned_equals_aligned(dev, precision):
+    """Every map functor, float and double, both policies: a table that is
+    not 16-byte aligned (the element-wise kernel) gives the same bits as the
+    aligned one (the vector kernel and its tail), specials included."""
+    for dtype, fns in ((np.float32, F32_OPS), (np.float64, F64_OPS)):
+        for fn in fns:
+            x = _args(fn, (1 << 16) + 3, dtype)
+            a = _gpu(x, dev)
+            u = _gpu(np.concatenate([[0], x]).astype(dtype), dev)[1:]
+            assert a.data_ptr() % 16 == 0 and u.data_ptr() % 16 != 0
+            ga = ops.map(fn, a, precision).cpu().numpy()
+            gu = ops.map(fn, u, precision).cpu().numpy()
+            eq = _bits_equal(ga, gu)
+            assert eq.all(), (dtype.__name__, fn, x[~eq][:4], ga[~eq][:4], gu[~eq][:4])
+
+
 def test_map_in_place_and_empty(dev, ref):
     import torch
 
