@@ -294,6 +294,20 @@ def test_map_f64_exact_every_binade(dev, ref, fn):
     assert eq.all(), (int((~eq).sum()), x[~eq][:8], got[~eq][:8], want[~eq][:8])
 
 
+@pytest.mark.parametrize("fn", F64_OPS)
+def test_map_f64_exact_random_bit_patterns(dev, ref, fn):
+    """EXACT double functors on 2^26 uniformly random 64-bit patterns (every
+    sign, exponent and mantissa equally likely: subnormals, infinities and
+    NaNs included): bit-identical to the oracle (NaN where the oracle has
+    NaN)."""
+    rng = np.random.default_rng(34 + F64_OPS.index(fn))
+    x = rng.integers(0, np.iinfo(np.uint64).max, 1 << 26, dtype=np.uint64, endpoint=True).view(np.float64)
+    got = ops.map(fn, _gpu(x, dev), nt2_amd.EXACT).cpu().numpy()
+    want = ref.map_f64(fn, x)
+    eq = _bits_equal(got, want)
+    assert eq.all(), (int((~eq).sum()), x[~eq][:8], got[~eq][:8], want[~eq][:8])
+
+
 # the branch points of NT2's double functors and reductions (math.hpp cites
 # each): exp's Maxlog, Minlog and 1023.5 ln2; erf/erfc intervals; tanh's
 # 0.625 and its saturation; sin/cos ladder bounds pi/4, pi/2, 20 pi,
