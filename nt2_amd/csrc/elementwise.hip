@@ -332,6 +332,15 @@ template <class P> __device__ __forceinline__ void ncdf_pair(float d, float& nd,
   nmd = 0.5f * ((t > 0.0f) ? rr : r);
   if (d != d) { nd = d; nmd = d; }
 }
+// the same with exp(-d^2/2) given (Fused policy)
+template <class P> __device__ __forceinline__ void ncdf_pair_e(float d, float e, float& nd, float& nmd) {
+  float t = -0x1.6a09e6p-1f * d;
+  float r = erfc_pos_e<P>(__builtin_fabsf(t), e);
+  float rr = 2.0f - r;
+  nd = 0.5f * ((t < 0.0f) ? rr : r);
+  nmd = 0.5f * ((t > 0.0f) ? rr : r);
+  if (d != d) { nd = d; nmd = d; }
+}
 template <class P> __device__ __forceinline__ void ncdf_pair(double d, double& nd, double& nmd) {
   double t = -0x1.6a09e667f3bcdp-1 * d;
   double r = erfc_pos_d<P>(__builtin_fabs(t));
@@ -348,6 +357,29 @@ template <class P> __device__ __forceinline__ double expT(double x) { return exp
 template <class T, class P> struct BSFwdOp {
   __device__ __forceinline__ void operator()(const T (&in)[5], T (&out)[2]) const {
     const T S = in[0], K = in[1], Tm = in[2], r = in[3], sig = in[4];
+    if constexpr (P::fused && sizeof(T) == 4) {
+      // exp(-d2^2/2) = exp(-d1^2/2) (S/K) e^{rT} (d2 = d1 - sig sqrt T), as in
+      // the adjoint kernel: one exp for both normcdf pairs, and more accurate
+      // than exp(-d2^2/2) of the rounded d2.  Where a factor left the float
+      // range (a product above 1: e^{-rT} underflowed or (S/K) e^{rT}
+      // overflowed; or e^{-rT} itself overflowed, which makes (S/K) e^{rT}
+      // vanish) exp(-d2^2/2) is formed from d2 instead (rare lanes)
+      const float sst = sig * P::sqrt(Tm);
+      const float sk = P::div(S, K);
+      const float d1 = P::div(P::madd(P::madd(sig, sig * 0.5f, r), Tm, logT<P>(sk)), sst);
+      const float d2 = d1 - sst;
+      const float ert = expT<P>(-r * Tm);
+      const float kert = K * ert;
+      const float e1 = exp_neg_half_square<P>(d1);
+      float e2 = e1 * (sk * P::rcp(ert));
+      if (!(e2 <= 2.0f) || !(ert <= 0x1.fffffep+127f)) e2 = exp_neg_half_square<P>(d2);
+      float n1, nm1, n2, nm2;
+      ncdf_pair_e<P>(d1, e1, n1, nm1);
+      ncdf_pair_e<P>(d2, e2, n2, nm2);
+      out[0] = S * n1 - kert * n2;
+      out[1] = kert * nm2 - S * nm1;
+      return;
+    }
     T sqT = P::sqrt(Tm);
     T sst = sig * sqT;
     T d1;
