@@ -250,8 +250,9 @@ int nt2h_fold_f64(int op, const double* x, int64_t lo, int64_t mid, int64_t hi, 
  * cummin (min, Inf) — boost/simd/swar/functions/{cumsum,cumprod,cummax,
  * cummin}.hpp.  cummax/cummin are bit-identical to NT2's recurrence (NaN
  * resets included); cumsum uses compensated float pairs with FP64 carries
- * (+-inf and NaN from where the recurrence reaches them) and cumprod an FP64 mantissa with a separate exponent (no intermediate
- * overflow).  y may equal x. */
+ * (+-inf and NaN from where the recurrence reaches them) and cumprod an
+ * FP64 mantissa with a separate exponent (no intermediate overflow).  y may
+ * equal x. */
 typedef enum nt2h_scan_op {
   NT2H_SCAN_CUMSUM = 0,
   NT2H_SCAN_CUMPROD = 1,
