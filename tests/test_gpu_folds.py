@@ -596,3 +596,49 @@ def test_prod_and_cumprod_specials(dev, dtype, shape):
     rtol = 1e-6 if dtype == np.float32 else 1e-9
     _same_ieee(_fold("prod", x, lo, mid, hi, dev), wp, rtol)
     _same_ieee(_scan("cumprod", x, lo, mid, hi, dev), wc, rtol)
+
+
+def _fold_scan_at(kind, op, x, lo, mid, hi, dev, offset):
+    """nt2h_fold/nt2h_scan with the input (and, for a scan, the output) placed
+    `offset` elements into its allocation (offset 1: not 16-byte aligned)."""
+    import torch
+
+    sfx = _sfx(x)
+    gx = _dev_tensor(np.concatenate([np.zeros(offset, x.dtype), x]), dev)[offset:]
+    nout = lo * hi if kind == "fold" else x.size
+    y = torch.zeros(nout + offset, dtype=gx.dtype, device=dev)[offset:]
+    if kind == "fold":
+        nt2_amd.call("nt2h_fold_" + sfx, nt2_amd.FOLD_OPS[op], gx.data_ptr(), lo, mid, hi, y.data_ptr(),
+                     torch.cuda.current_stream().cuda_stream)
+    else:
+        nt2_amd.call("nt2h_scan_" + sfx, nt2_amd.SCAN_OPS[op], gx.data_ptr(), lo, mid, hi, y.data_ptr(),
+                     torch.cuda.current_stream().cuda_stream)
+    return y.cpu().numpy()
+
+
+@pytest.mark.parametrize("shape", [(1, 5000, 37), (7, 3000, 5), (16, 5000, 3), (1, (1 << 20) + 3, 1), (4, 30000, 1),
+                                   (64, 4097, 1), (1, 1 << 18, 2), (1024, 300, 2), (3, 5, 2)])
+def test_folds_and_scans_of_unaligned_tables(dev, ref, shape):
+    """Input and output one element past a 16-byte boundary (the element-wise
+    loads and stores of every fold and scan path): maximum, minimum, cummax
+    and cummin bit-identical to NT2's recurrence, the sums and products and
+    cumsum/cumprod within 1e-6 of the FP64 values, as for aligned tables."""
+    lo, mid, hi = shape
+    x = _specials_data(lo * mid * hi, 51)
+    for op in ("maximum", "minimum"):
+        got = _fold_scan_at("fold", op, x, lo, mid, hi, dev, 1)
+        assert _bits_eq(got, ref.fold_lmh(op, x, lo, mid, hi)).all(), op
+    for op in ("cummax", "cummin"):
+        got = _fold_scan_at("scan", op, x, lo, mid, hi, dev, 1)
+        assert _bits_eq(got, ref.scan_lmh(op, x, lo, mid, hi)).all(), op
+    v = np.random.default_rng(52).uniform(0.5, 1.5, lo * mid * hi).astype(np.float32)
+    v3 = v.astype(np.float64).reshape(hi, mid, lo)
+    p = np.random.default_rng(53).uniform(0.999, 1.001, lo * mid * hi).astype(np.float32)
+    p3 = p.astype(np.float64).reshape(hi, mid, lo)
+    checks = (("fold", "sum", v, v3.sum(axis=1).reshape(-1)), ("fold", "prod", p, p3.prod(axis=1).reshape(-1)),
+              ("scan", "cumsum", v, np.cumsum(v3, axis=1).reshape(-1)),
+              ("scan", "cumprod", p, np.cumprod(p3, axis=1).reshape(-1)))
+    for kind, op, data, want in checks:
+        for offset in (0, 1):
+            got = _fold_scan_at(kind, op, data, lo, mid, hi, dev, offset).astype(np.float64)
+            assert np.allclose(got, want, rtol=1e-6, atol=0), (kind, op, offset, np.max(np.abs(got / want - 1)))
