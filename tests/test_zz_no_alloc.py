@@ -98,6 +98,35 @@ def test_noalloc_refuses_host_totals_and_handles_empty(dev):
         assert torch.equal(a.view(torch.int32), b.view(torch.int32))
 
 
+def test_noalloc_unaligned_tables(dev):
+    """Tables one element past a 16-byte boundary (the element-wise kernel,
+    whose grid the entry point caps to its scratch): the six streams
+    bit-identical to nt2h_bs_fwd_adj_f32's on the same tables, the totals
+    (summed over a different number of workgroups) within 1e-12 of the sum
+    of |terms| of each, on two calls in a row, the second smaller."""
+    import torch
+
+    for n in (100003, 4101):
+        d = synth.bs_inputs(n, seed=5, with_w=True)
+        ins = [torch.from_numpy(np.concatenate([[0], d[k]]).astype(np.float32)).to(dev)[1:]
+               for k in ("S", "K", "T", "r", "sig", "w")]
+        assert all(t.data_ptr() % 16 != 0 for t in ins)
+        res = {}
+        for fn in ("nt2h_bs_fwd_adj_f32", "nt2h_bs_fwd_adj_f32_noalloc"):
+            outs = [torch.full((n + 1,), float("nan"), dtype=torch.float32, device=dev)[1:] for _ in range(6)]
+            tot = torch.full((2,), float("nan"), dtype=torch.float64, device=dev)
+            print(f"{fn} unaligned n={n}", flush=True)
+            assert _call(fn, ins, outs, tot, n, torch.cuda.current_stream()) == 0
+            torch.cuda.synchronize()
+            res[fn] = ([o.cpu().numpy() for o in outs], tot.cpu().numpy())
+        (o1, t1), (o2, t2) = res["nt2h_bs_fwd_adj_f32"], res["nt2h_bs_fwd_adj_f32_noalloc"]
+        for a, b in zip(o1, o2):
+            assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+        for k, j in ((0, 4), (1, 5)):
+            mag = np.abs(o1[j].astype(np.float64)).sum()
+            assert abs(t1[k] - t2[k]) <= 1e-12 * mag, (k, t1[k], t2[k])
+
+
 def test_noalloc_recorded_graph_replays(dev):
     import torch
 
