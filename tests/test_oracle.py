@@ -176,6 +176,27 @@ def test_exp_flushes_at_k_minus_127(ref):
     assert np.any((v[k == -126] > 0) & (v[k == -126] < np.finfo(np.float32).tiny))
 
 
+def test_double_quirks_of_nt2(ref):
+    """NT2's double path as its sources define it, which the oracle (and the
+    EXACT kernels) keep: exp returns inf once k = round(x/ln2) reaches 1024
+    below Maxlog, because scale (expo_scale.hpp:27-35) builds the exponent
+    field 2047 (d_expo_reduction.hpp:47-49: x >= Maxlog = 709.78 only); log reads a
+    subnormal's exponent field through fast_frexp (d_log.hpp:65), so its
+    result is far off; erf's x > 6 branch returns 1 whatever the sign
+    (scalar/erf.hpp:69).  Each is contrasted with the neighbouring inputs that
+    NT2 gets right."""
+    ln2 = math.log(2.0)
+    x = np.array([1023.4 * ln2, 1023.6 * ln2, 709.78], np.float64)
+    e = ref.map_f64("exp", x)
+    assert np.isfinite(e[0]) and abs(e[0] / math.exp(x[0]) - 1) < 1e-15
+    assert e[1] == np.inf and e[2] == np.inf  # finite true values: NT2's k = 1024
+    lg = ref.map_f64("log", np.array([2.2250738585072014e-308, 1e-310], np.float64))
+    assert abs(lg[0] - math.log(2.2250738585072014e-308)) < 1e-12
+    assert abs(lg[1] - math.log(1e-310)) > 1.0  # the subnormal misread
+    ef = ref.map_f64("erf", np.array([-5.9, -6.5, 6.5], np.float64))
+    assert ef[0] < -0.99 and ef[1] == 1.0 and ef[2] == 1.0
+
+
 # ------------------------------------------------------- mpmath accuracy bounds
 # std-ULP against the true value (|ref| floored at FLT_MIN); NT2's documented
 # or tested accuracies: exp/log 0.5 NT2-ULP (1 std-ULP of float(f(double)),
