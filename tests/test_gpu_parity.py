@@ -689,6 +689,25 @@ def test_bs_fwd_adj_config4_full_size(dev, ref):
         assert abs(tot[k] - (hi + lo)) <= 1e-13 * np.sum(np.abs(got[j]).astype(np.float64)), k
 
 
+def test_bs_fwd_f64_fused_close_to_exact(dev, ref):
+    """The FUSED double forward (NT2's double functors with FMA) against the
+    EXACT one (NT2's double path, bit-identical to the oracle) on the C2
+    distributions and on the wide inputs: scaled error |dP| / max(|P|, 1e-6 S)
+    below 1e-12 where both are finite, and finite exactly where EXACT is."""
+    d = synth.bs_inputs(1 << 18, seed=7)
+    sets = [[d[k].astype(np.float64) for k in ("S", "K", "T", "r", "sig")], _bs_wide_inputs((1 << 14) + 1, np.float64)]
+    for h in sets:
+        g = [_gpu(a, dev) for a in h]
+        ce, pe = [t.cpu().numpy() for t in ops.bs_fwd(*g, precision=nt2_amd.EXACT)]
+        cf, pf = [t.cpu().numpy() for t in ops.bs_fwd(*g, precision=nt2_amd.FUSED)]
+        S = np.abs(h[0])
+        for a, b in ((cf, ce), (pf, pe)):
+            assert (np.isfinite(a) == np.isfinite(b)).all()
+            m = np.isfinite(b) & (S < 1e30) & (S > 1e-30)
+            err = np.abs(a[m] - b[m]) / np.maximum(np.abs(b[m]), 1e-6 * S[m])
+            assert err.max() < 1e-12, err.max()
+
+
 def test_bs_unaligned_tables(dev, ref):
     """Black-Scholes forward (EXACT, float and double) and forward+adjoint
     with one table one element past a 16-byte boundary, which sends the whole
