@@ -63,3 +63,41 @@ def test_bs_host_pipeline_empty(dev):
     z = np.zeros(1, np.float32)
     nt2_amd.call("nt2h_bs_fwd_adj_f32_host", *[z.ctypes.data] * 12, tot.ctypes.data, 0, 0)
     assert tot[0] == 0 and tot[1] == 0
+
+
+@pytest.mark.parametrize("nbytes", [0, 1, 4099, (3 << 20) + 7])
+def test_pinned_and_device_buffers_with_memcpy(dev, nbytes):
+    """nt2h_host_alloc (pinned) -> nt2h_memcpy H2D into nt2h_malloc memory ->
+    D2D -> D2H and H2H into more pinned memory: the bytes arrive unchanged for
+    every copy kind (and DEFAULT), zero-byte copies included; every buffer is
+    released through its own free. Sizes past 1 MiB take nt2h_malloc's offset
+    (coloured) allocations, whose free must find the base again."""
+    L = nt2_amd.lib()
+    s = C.c_void_p(_stream())
+    if nbytes == 0:  # a zero-byte request succeeds with a null pointer
+        for alloc in (L.nt2h_malloc, L.nt2h_host_alloc):
+            z = C.c_void_p(1)
+            assert alloc(C.byref(z), 0) == 0 and not z.value
+    hp = [C.c_void_p() for _ in range(3)]
+    dp = [C.c_void_p() for _ in range(2)]
+    for p in hp:
+        assert L.nt2h_host_alloc(C.byref(p), max(nbytes, 1)) == 0 and p.value
+    for p in dp:
+        assert L.nt2h_malloc(C.byref(p), max(nbytes, 1)) == 0 and p.value
+    try:
+        src = np.random.default_rng(nbytes).integers(0, 256, nbytes, dtype=np.uint8)
+        C.memmove(hp[0].value, src.ctypes.data, nbytes)
+        assert L.nt2h_memcpy(dp[0], hp[0], nbytes, 1, s) == 0  # H2D
+        assert L.nt2h_memcpy(dp[1], dp[0], nbytes, 3, s) == 0  # D2D
+        assert L.nt2h_memcpy(hp[1], dp[1], nbytes, 2, s) == 0  # D2H
+        assert L.nt2h_memcpy(hp[2], hp[1], nbytes, 0, s) == 0  # H2H
+        assert L.nt2h_memcpy(hp[1], dp[0], nbytes, 4, s) == 0  # DEFAULT (inferred)
+        assert L.nt2h_sync(s) == 0
+        for p in hp[1:]:
+            got = np.frombuffer((C.c_uint8 * nbytes).from_address(p.value), np.uint8) if nbytes else np.zeros(0, np.uint8)
+            assert np.array_equal(got, src)
+    finally:
+        for p in hp:
+            assert L.nt2h_host_free(p) == 0
+        for p in dp:
+            assert L.nt2h_free(p) == 0
