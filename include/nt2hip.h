@@ -123,7 +123,10 @@ int nt2h_last_hip_error(void);          /* hipError_t of the last failing HIP ca
  * successive 4 KiB "colours" modulo 64 KiB, so that the tables one loop nest
  * streams together do not share an offset modulo the 2 MiB allocation
  * alignment (same HBM channel and bank group).  nt2h_free takes exactly the
- * pointer nt2h_malloc returned. */
+ * pointer nt2h_malloc returned.  A zero-byte request sets *ptr to NULL and
+ * returns NT2H_OK; freeing NULL is a no-op.  nt2h_memcpy is asynchronous on
+ * `s` (kind is an nt2h_copy_kind; NT2H_DEFAULT infers the direction from the
+ * pointers); a zero-byte copy is a no-op, an unknown kind NT2H_ERR_INVALID_ARG. */
 int nt2h_malloc(void** ptr, size_t bytes);
 int nt2h_free(void* ptr);
 int nt2h_host_alloc(void** ptr, size_t bytes); /* pinned host memory */
