@@ -286,11 +286,16 @@ def test_long_run_scans(dev, ref, shape):
 
 
 @pytest.mark.parametrize("shape", [(1, 1 << 19, 1), (1, (1 << 16) + 4, 3), (1, 20000, 1), (1, 16384, 2),
-                                   (1, 1 << 19, 4)])
+                                   (1, 1 << 19, 4),
+                                   # either side of the single-launch limits: 128 tiles of
+                                   # 8192 floats per run (64 for cumprod) and 512 in all
+                                   (1, 1 << 20, 1), (1, (1 << 20) + 1, 1), (1, (1 << 19) + 1, 1),
+                                   (1, 1 << 20, 4), (1, 1 << 20, 5), (1, 1 << 19, 9)])
 def test_lookback_scans(dev, ref, shape):
-    """Runs of 2^14 to 2^19 floats are scanned in one launch, tiles finding
+    """Runs of 2^14 to 2^20 floats are scanned in one launch, tiles finding
     their prefix from the published state of the tiles before them (a
-    partial last tile, several runs).  cummax/cummin bit-identical to NT2's
+    partial last tile, several runs); one tile more per run or in all takes
+    the three passes instead.  cummax/cummin bit-identical to NT2's
     recurrence, cumsum/cumprod within the bounds of the other paths, and the
     bits do not depend on the timing: three calls agree."""
     lo, mid, hi = shape
